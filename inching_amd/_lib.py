@@ -1,0 +1,78 @@
+"""ctypes binding of libinching_hip.so (the C ABI declared in include/inching.h).
+
+The library is required: importing this module without the built library raises
+(there is no CPU fallback anywhere in the product path).
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libinching_hip.so")
+
+F32, F64 = 0, 1
+
+_i32, _i64, _u64, _dbl, _vp = C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_void_p
+_STATUS = C.c_int32
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "inch_status_string": (C.c_char_p, [_i32]),
+    "inch_last_error": (C.c_char_p, []),
+    "inch_abi_version": (_i32, []),
+    "inch_synth_globule": (_STATUS, [_i64, _dbl, _dbl, _u64, _i64, _vp]),
+    "inch_rcm_host": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _vp, _vp]),
+    "inch_batch_head": (_STATUS, [_i64, _i64, _vp, _i64, _vp]),
+    "inch_neighbors_celllist": (_STATUS, [_i64, _vp, _dbl, _vp, _vp, _i64, _vp, _vp]),
+    "inch_hessian_assemble": (_STATUS, [_i64, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_bsr_spmv": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp, _dbl,
+                                _vp, _vp, _i32, _i32, _vp]),
+    "inch_bsr_spmv_part": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp]),
+    "inch_bsr_spmm": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp]),
+    "inch_anm_matfree_apply": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _vp, _vp, _i32, _vp]),
+    "inch_multidot_work_size": (_i64, [_i32]),
+    "inch_multidot": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "inch_gemv_update": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _dbl, _vp, _vp, _vp, _vp]),
+    "inch_vec_scale_div": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    "inch_vec_axpy_dev": (_STATUS, [_i64, _vp, _dbl, _vp, _vp, _vp]),
+    "inch_vec_convert": (_STATUS, [_i64, _vp, _i32, _vp, _i32, _vp]),
+    "inch_vec_axpby": (_STATUS, [_i64, _dbl, _vp, _i32, _dbl, _vp, _i32, _vp]),
+    "inch_rr_combine": (_STATUS, [_i64, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
+}
+
+
+class InchingError(RuntimeError):
+    def __init__(self, name, status, msg):
+        super().__init__("%s failed (%d): %s" % (name, status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libinching_hip.so is not built (expected at %s); run "
+                              "`python -m inching_amd.build`" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    """Call an ``inch_*`` entry point and raise InchingError on a non-zero status."""
+    fn = getattr(lib(), name)
+    st = fn(*args)
+    if SIGNATURES[name][0] is _STATUS and st != 0:
+        msg = lib().inch_last_error()
+        raise InchingError(name, st, msg.decode() if msg else "")
+    return st
+
+
+def exported_symbols():
+    return sorted(SIGNATURES)

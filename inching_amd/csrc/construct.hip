@@ -1,0 +1,373 @@
+// Construction kernels for gfx950: cell-list neighbour search and ANM 3x3
+// super-element assembly into full-storage block rows.
+//
+// Neighbour rule (shared with assembly and the matrix-free operator):
+//   j in ball(i)  <=>  (dx*dx + dy*dy) + dz*dz <= rc*rc,  dx = x_j - x_i (float64),
+// evaluated with explicit round-to-nearest operations so no FMA contraction can
+// change the set.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+constexpr int kMaxRow = 2048;  // capacity of one neighbour row in the sort kernel
+
+__device__ __forceinline__ double dist2(double dx, double dy, double dz) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+struct Grid {
+  double lo[3];
+  double inv;  // 1 / cell size
+  int64_t dim[3];
+};
+
+__device__ __forceinline__ int64_t cell_coord(double v, double lo, double inv, int64_t dim) {
+  int64_t c = (int64_t)floor((v - lo) * inv);
+  return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
+}
+
+__global__ void k_minmax(int64_t n, const double *__restrict__ xyz, double *__restrict__ part) {
+  // part[blockIdx.x*6 + {0..2}] = min, {3..5} = max
+  __shared__ double smin[3][256], smax[3][256];
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    for (int a = 0; a < 3; ++a) {
+      double v = xyz[3 * i + a];
+      mn[a] = fmin(mn[a], v);
+      mx[a] = fmax(mx[a], v);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    smin[a][threadIdx.x] = mn[a];
+    smax[a][threadIdx.x] = mx[a];
+  }
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int a = 0; a < 3; ++a) {
+        smin[a][threadIdx.x] = fmin(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+        smax[a][threadIdx.x] = fmax(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 3; ++a) {
+      part[blockIdx.x * 6 + a] = smin[a][0];
+      part[blockIdx.x * 6 + 3 + a] = smax[a][0];
+    }
+}
+
+__global__ void k_cell_count(int64_t n, const double *__restrict__ xyz, Grid g, int32_t *__restrict__ cell_of,
+                             int32_t *__restrict__ count) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t c[3];
+  for (int a = 0; a < 3; ++a) c[a] = cell_coord(xyz[3 * i + a], g.lo[a], g.inv, g.dim[a]);
+  int64_t id = (c[0] * g.dim[1] + c[1]) * g.dim[2] + c[2];
+  cell_of[i] = (int32_t)id;
+  atomicAdd(&count[id], 1);
+}
+
+// exclusive scan of int32 counts into int32 starts (single block, chunked)
+__global__ void k_scan_i32(int64_t m, const int32_t *__restrict__ in, int32_t *__restrict__ out) {
+  __shared__ int32_t buf[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < m; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int32_t v = i < m ? in[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int32_t t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < m) out[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = carry;
+}
+
+__global__ void k_cell_fill(int64_t n, const int32_t *__restrict__ cell_of, const int32_t *__restrict__ start,
+                            int32_t *__restrict__ cursor, int32_t *__restrict__ atoms) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int32_t c = cell_of[i];
+  int32_t p = atomicAdd(&cursor[c], 1);
+  atoms[start[c] + p] = (int32_t)i;
+}
+
+// One thread per atom: count (indices == nullptr) or write neighbours in cell order.
+__global__ void k_neigh(int64_t n, const double *__restrict__ xyz, Grid g, double rc2,
+                        const int32_t *__restrict__ start, const int32_t *__restrict__ atoms,
+                        const int64_t *__restrict__ indptr, int64_t *__restrict__ counts,
+                        int32_t *__restrict__ indices) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+  int64_t c0 = cell_coord(xi, g.lo[0], g.inv, g.dim[0]);
+  int64_t c1 = cell_coord(yi, g.lo[1], g.inv, g.dim[1]);
+  int64_t c2 = cell_coord(zi, g.lo[2], g.inv, g.dim[2]);
+  int64_t cnt = 0;
+  int64_t out = indices ? indptr[i] : 0;
+  int64_t end = indices ? indptr[i + 1] : 0;
+  for (int64_t a = c0 - 1; a <= c0 + 1; ++a) {
+    if (a < 0 || a >= g.dim[0]) continue;
+    for (int64_t b = c1 - 1; b <= c1 + 1; ++b) {
+      if (b < 0 || b >= g.dim[1]) continue;
+      for (int64_t c = c2 - 1; c <= c2 + 1; ++c) {
+        if (c < 0 || c >= g.dim[2]) continue;
+        int64_t id = (a * g.dim[1] + b) * g.dim[2] + c;
+        for (int32_t p = start[id]; p < start[id + 1]; ++p) {
+          int32_t j = atoms[p];
+          double d2 = dist2(__dsub_rn(xyz[3 * (int64_t)j], xi), __dsub_rn(xyz[3 * (int64_t)j + 1], yi),
+                            __dsub_rn(xyz[3 * (int64_t)j + 2], zi));
+          if (d2 <= rc2) {
+            if (indices) {
+              if (out < end) indices[out] = j;
+              ++out;
+            }
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  if (!indices) counts[i] = cnt;
+}
+
+// exclusive scan of int64 counts -> indptr (single block)
+__global__ void k_scan_i64(int64_t m, const int64_t *__restrict__ in, int64_t *__restrict__ out) {
+  __shared__ int64_t buf[1024];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < m; base += 1024) {
+    int64_t i = base + threadIdx.x;
+    int64_t v = i < m ? in[i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int64_t t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < m) out[i] = carry + buf[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += buf[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[m] = carry;
+}
+
+// One block (256 threads) per row: bitonic sort of the row's columns in LDS.
+__global__ void __launch_bounds__(256) k_sort_rows(int64_t n, const int64_t *__restrict__ indptr,
+                                                    int32_t *__restrict__ indices, int32_t *__restrict__ overflow) {
+  __shared__ int32_t buf[kMaxRow];
+  int64_t row = blockIdx.x;
+  if (row >= n) return;
+  int64_t b = indptr[row], e = indptr[row + 1];
+  int len = (int)(e - b);
+  if (len > kMaxRow) {
+    if (threadIdx.x == 0) atomicMax(overflow, len);
+    return;
+  }
+  int p2 = 1;
+  while (p2 < len) p2 <<= 1;
+  for (int t = threadIdx.x; t < p2; t += blockDim.x) buf[t] = t < len ? indices[b + t] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= p2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < p2; t += blockDim.x) {
+        int ixj = t ^ j;
+        if (ixj > t) {
+          bool up = (t & k) == 0;
+          int32_t a = buf[t], c = buf[ixj];
+          if ((a > c) == up) {
+            buf[t] = c;
+            buf[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int t = threadIdx.x; t < len; t += blockDim.x) indices[b + t] = buf[t];
+}
+
+// One 64-lane wave per row: H_ij = -d d^T / r^2 for j != i (r^2 > 0); the diagonal
+// block is minus the sum of the row's off-diagonal blocks plus plus_i * I.
+__global__ void __launch_bounds__(256) k_assemble(int64_t n, const double *__restrict__ xyz, double plus_i,
+                                                   const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                   double *__restrict__ v64, float *__restrict__ v32,
+                                                   double *__restrict__ diag) {
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (row >= n) return;
+  const double xi = xyz[3 * row], yi = xyz[3 * row + 1], zi = xyz[3 * row + 2];
+  double s[6] = {0, 0, 0, 0, 0, 0};  // xx xy xz yy yz zz of sum d d^T / r^2
+  int64_t selfpos = -1;
+  const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
+  for (int64_t b = b0 + lane; b < b1; b += 64) {
+    int32_t j = col[b];
+    double h[9];
+    if (j == row) {
+      selfpos = b;
+      continue;
+    }
+    double dx = __dsub_rn(xyz[3 * (int64_t)j], xi), dy = __dsub_rn(xyz[3 * (int64_t)j + 1], yi),
+           dz = __dsub_rn(xyz[3 * (int64_t)j + 2], zi);
+    double r2 = dist2(dx, dy, dz);
+    double c = r2 > 0.0 ? -1.0 / r2 : 0.0;
+    double d[3] = {dx, dy, dz};
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) h[3 * p + q] = d[p] * d[q] * c;
+    s[0] -= h[0];
+    s[1] -= h[1];
+    s[2] -= h[2];
+    s[3] -= h[4];
+    s[4] -= h[5];
+    s[5] -= h[8];
+    if (v64)
+      for (int t = 0; t < 9; ++t) v64[9 * b + t] = h[t];
+    if (v32)
+      for (int t = 0; t < 9; ++t) v32[9 * b + t] = (float)h[t];
+  }
+  // deterministic butterfly reduction over the wave
+  for (int off = 32; off > 0; off >>= 1)
+    for (int t = 0; t < 6; ++t) s[t] += __shfl_xor(s[t], off, 64);
+  // the lane holding the self block writes it
+  if (selfpos >= 0) {
+    double h[9] = {s[0] + plus_i, s[1], s[2], s[1], s[3] + plus_i, s[4], s[2], s[4], s[5] + plus_i};
+    if (v64)
+      for (int t = 0; t < 9; ++t) v64[9 * selfpos + t] = h[t];
+    if (v32)
+      for (int t = 0; t < 9; ++t) v32[9 * selfpos + t] = (float)h[t];
+  }
+  if (diag && lane == 0) {
+    diag[3 * row] = s[0] + plus_i;
+    diag[3 * row + 1] = s[3] + plus_i;
+    diag[3 * row + 2] = s[5] + plus_i;
+  }
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) INCH_FAIL(INCH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, double rc, int64_t *indptr,
+                                               int32_t *indices, int64_t capacity, int64_t *nnz_out,
+                                               void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && indptr && rc > 0, "inch_neighbors_celllist: bad args");
+  INCH_REQUIRE(n < (int64_t)1 << 31, "inch_neighbors_celllist: n must fit int32");
+  hipStream_t st = (hipStream_t)stream;
+  // bounds
+  const int nb = 256;
+  DevBuf part;
+  HIP_TRY(hipMalloc(&part.p, nb * 6 * sizeof(double)));
+  k_minmax<<<nb, 256, 0, st>>>(n, xyz, (double *)part.p);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> hp(nb * 6);
+  HIP_TRY(hipMemcpyAsync(hp.data(), part.p, nb * 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  Grid g;
+  double hi[3];
+  for (int a = 0; a < 3; ++a) {
+    g.lo[a] = INFINITY;
+    hi[a] = -INFINITY;
+    for (int b = 0; b < nb; ++b) {
+      g.lo[a] = std::min(g.lo[a], hp[b * 6 + a]);
+      hi[a] = std::max(hi[a], hp[b * 6 + 3 + a]);
+    }
+    if (!std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_neighbors_celllist: non-finite coordinates");
+  }
+  g.inv = 1.0 / rc;
+  int64_t ncell = 1;
+  for (int a = 0; a < 3; ++a) {
+    g.dim[a] = std::max<int64_t>(1, (int64_t)std::floor((hi[a] - g.lo[a]) * g.inv) + 1);
+    ncell *= g.dim[a];
+  }
+  if (ncell > ((int64_t)1 << 30)) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist: %lld cells", (long long)ncell);
+  DevBuf cell_of, cnt, start, atoms, counts, ovf;
+  HIP_TRY(hipMalloc(&cell_of.p, n * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&cnt.p, (ncell + 1) * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&start.p, (ncell + 1) * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&atoms.p, n * sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
+  const int bs = 256;
+  const int64_t gb = (n + bs - 1) / bs;
+  k_cell_count<<<gb, bs, 0, st>>>(n, xyz, g, (int32_t *)cell_of.p, (int32_t *)cnt.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_i32<<<1, 1024, 0, st>>>(ncell, (int32_t *)cnt.p, (int32_t *)start.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
+  k_cell_fill<<<gb, bs, 0, st>>>(n, (int32_t *)cell_of.p, (int32_t *)start.p, (int32_t *)cnt.p, (int32_t *)atoms.p);
+  HIP_TRY(hipGetLastError());
+  const double rc2 = rc * rc;
+  if (indices == nullptr) {
+    HIP_TRY(hipMalloc(&counts.p, n * sizeof(int64_t)));
+    k_neigh<<<gb, bs, 0, st>>>(n, xyz, g, rc2, (int32_t *)start.p, (int32_t *)atoms.p, nullptr, (int64_t *)counts.p,
+                               nullptr);
+    HIP_TRY(hipGetLastError());
+    k_scan_i64<<<1, 1024, 0, st>>>(n, (int64_t *)counts.p, indptr);
+    HIP_TRY(hipGetLastError());
+    int64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nnz_out) *nnz_out = total;
+    return INCH_OK;
+  }
+  int64_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (total > capacity) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist: need %lld > capacity %lld", (long long)total, (long long)capacity);
+  k_neigh<<<gb, bs, 0, st>>>(n, xyz, g, rc2, (int32_t *)start.p, (int32_t *)atoms.p, indptr, nullptr, indices);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMalloc(&ovf.p, sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(ovf.p, 0, sizeof(int32_t), st));
+  k_sort_rows<<<n, 256, 0, st>>>(n, indptr, indices, (int32_t *)ovf.p);
+  HIP_TRY(hipGetLastError());
+  int32_t over = 0;
+  HIP_TRY(hipMemcpyAsync(&over, ovf.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (over) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist: a row has %d neighbours (> %d)", over, kMaxRow);
+  if (nnz_out) *nnz_out = total;
+  return INCH_OK;
+}
+
+extern "C" inch_status inch_hessian_assemble(int64_t n, const double *xyz, double rc, double plus_i,
+                                             const int64_t *rowptr, const int32_t *col, double *vals64,
+                                             float *vals32, double *diag, void *stream) {
+  (void)rc;  // the pattern already encodes the cutoff; only r^2 == 0 is masked here
+  INCH_REQUIRE(n > 0 && xyz && rowptr && col, "inch_hessian_assemble: bad args");
+  INCH_REQUIRE(vals64 || vals32 || diag, "inch_hessian_assemble: no output");
+  hipStream_t st = (hipStream_t)stream;
+  const int bs = 256;
+  const int64_t grid = (n * 64 + bs - 1) / bs;
+  k_assemble<<<grid, bs, 0, st>>>(n, xyz, plus_i, rowptr, col, vals64, vals32, diag);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}

@@ -1,0 +1,536 @@
+// Operator kernels for gfx950: full-storage 3x3 block-row SpMV / SpMM with fused
+// epilogues (shift, Chebyshev three-term step, Hotelling term), the matrix-free
+// FP64 ANM operator, and the FP64 dense kernels of the Lanczos / Rayleigh-Ritz
+// steps (multi-dot, rank-J update, combine).
+//
+// Layout: row i of the block matrix is handled by a group of G lanes of one
+// 64-lane wavefront (G in {4,8,16,32,64}); each lane streams every G-th 3x3 block
+// of the row (36 B fp32 / 72 B fp64 contiguous per block, consecutive lanes read
+// consecutive blocks) and gathers the 3 matching x entries; the group reduces its
+// 3 partial sums with a fixed xor butterfly (deterministic) and lanes 0..2 of the
+// group apply the epilogue to the row's 3 dofs.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) INCH_FAIL(INCH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+namespace {
+
+constexpr int kRedBlocks = 512;  // fixed grid of the two-phase reductions (determinism)
+
+template <typename MT, typename VT, int G>
+__global__ void __launch_bounds__(256) k_bsr_spmv(int64_t n, const int64_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ col, const MT *__restrict__ vals,
+                                                   const VT *__restrict__ x, VT *__restrict__ y, MT s1, MT c0,
+                                                   const VT *__restrict__ z, MT c2, VT *__restrict__ acc, MT mu,
+                                                   const double *__restrict__ Q, const double *__restrict__ qcoef,
+                                                   int mq) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  if (row >= n) return;  // whole groups leave together (a group never spans two rows)
+  const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
+  MT a0 = 0, a1 = 0, a2 = 0;
+  int64_t b = b0 + lane;
+  for (; b + G < b1; b += 2 * G) {
+    const int64_t ja = col[b], jb = col[b + G];
+    const MT *va = vals + 9 * b;
+    const MT *vb = vals + 9 * (b + G);
+    MT xa0 = (MT)x[3 * ja], xa1 = (MT)x[3 * ja + 1], xa2 = (MT)x[3 * ja + 2];
+    MT xb0 = (MT)x[3 * jb], xb1 = (MT)x[3 * jb + 1], xb2 = (MT)x[3 * jb + 2];
+    a0 += va[0] * xa0 + va[1] * xa1 + va[2] * xa2;
+    a1 += va[3] * xa0 + va[4] * xa1 + va[5] * xa2;
+    a2 += va[6] * xa0 + va[7] * xa1 + va[8] * xa2;
+    a0 += vb[0] * xb0 + vb[1] * xb1 + vb[2] * xb2;
+    a1 += vb[3] * xb0 + vb[4] * xb1 + vb[5] * xb2;
+    a2 += vb[6] * xb0 + vb[7] * xb1 + vb[8] * xb2;
+  }
+  if (b < b1) {
+    const int64_t ja = col[b];
+    const MT *va = vals + 9 * b;
+    MT xa0 = (MT)x[3 * ja], xa1 = (MT)x[3 * ja + 1], xa2 = (MT)x[3 * ja + 2];
+    a0 += va[0] * xa0 + va[1] * xa1 + va[2] * xa2;
+    a1 += va[3] * xa0 + va[4] * xa1 + va[5] * xa2;
+    a2 += va[6] * xa0 + va[7] * xa1 + va[8] * xa2;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, G);
+    a1 += __shfl_xor(a1, off, G);
+    a2 += __shfl_xor(a2, off, G);
+  }
+  if (lane < 3) {
+    const int64_t dof = 3 * row + lane;
+    const MT w = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+    MT out = w;
+    if (c0 != (MT)0) out -= c0 * (MT)x[dof];
+    out *= s1;
+    if (z) out -= c2 * (MT)z[dof];
+    if (qcoef) {
+      double q = 0.0;
+      const int64_t nd = 3 * n;
+      for (int m = 0; m < mq; ++m) q += qcoef[m] * Q[m * nd + dof];
+      out += (MT)q;
+    }
+    y[dof] = (VT)out;
+    if (acc) acc[dof] = (VT)((MT)acc[dof] + mu * out);
+  }
+}
+
+template <typename MT, typename VT, int G, int R>
+__global__ void __launch_bounds__(256) k_bsr_spmm(int64_t n, const int64_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ col, const MT *__restrict__ vals,
+                                                   const VT *__restrict__ X, VT *__restrict__ Y, int nrhs) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int r0 = blockIdx.y * R;
+  if (row >= n) return;
+  const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
+  MT a[3][R];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < R; ++r) a[c][r] = 0;
+  for (int64_t b = b0 + lane; b < b1; b += G) {
+    const int64_t j = col[b];
+    const MT *v = vals + 9 * b;
+    MT h[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) h[t] = v[t];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r0 + r < nrhs) {
+        MT x0 = (MT)X[(3 * j) * nrhs + r0 + r], x1 = (MT)X[(3 * j + 1) * nrhs + r0 + r],
+           x2 = (MT)X[(3 * j + 2) * nrhs + r0 + r];
+        a[0][r] += h[0] * x0 + h[1] * x1 + h[2] * x2;
+        a[1][r] += h[3] * x0 + h[4] * x1 + h[5] * x2;
+        a[2][r] += h[6] * x0 + h[7] * x1 + h[8] * x2;
+      }
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[c][r] += __shfl_xor(a[c][r], off, G);
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (r0 + r < nrhs) Y[(3 * row + c) * nrhs + r0 + r] = (VT)a[c][r];
+  }
+}
+
+// y_i = plus_i x_i + sum_{j != i, r2 > 0} d (d . (x_i - x_j)) / r2,  d = p_j - p_i
+template <int G>
+__global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__restrict__ xyz,
+                                                  const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                  double plus_i, const double *__restrict__ X, double *__restrict__ Y,
+                                                  int nrhs) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int r = blockIdx.y;
+  if (row >= n) return;
+  const double pi0 = xyz[3 * row], pi1 = xyz[3 * row + 1], pi2 = xyz[3 * row + 2];
+  const double vi0 = X[(3 * row) * nrhs + r], vi1 = X[(3 * row + 1) * nrhs + r], vi2 = X[(3 * row + 2) * nrhs + r];
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t b = rowptr[row] + lane; b < rowptr[row + 1]; b += G) {
+    const int64_t j = col[b];
+    if (j == row) continue;
+    const double d0 = xyz[3 * j] - pi0, d1 = xyz[3 * j + 1] - pi1, d2 = xyz[3 * j + 2] - pi2;
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (!(r2 > 0.0)) continue;
+    const double t = (d0 * (vi0 - X[(3 * j) * nrhs + r]) + d1 * (vi1 - X[(3 * j + 1) * nrhs + r]) +
+                      d2 * (vi2 - X[(3 * j + 2) * nrhs + r])) / r2;
+    a0 += d0 * t;
+    a1 += d1 * t;
+    a2 += d2 * t;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, G);
+    a1 += __shfl_xor(a1, off, G);
+    a2 += __shfl_xor(a2, off, G);
+  }
+  if (lane < 3) {
+    const double w = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
+    const double vi = lane == 0 ? vi0 : (lane == 1 ? vi1 : vi2);
+    Y[(3 * row + lane) * nrhs + r] = w + plus_i * vi;
+  }
+}
+
+// ---------------------------------------------------------------- dense FP64 ---
+
+template <int JT>
+__global__ void __launch_bounds__(256) k_multidot_part(int64_t len, int J, const double *__restrict__ V, int64_t ld,
+                                                        const double *__restrict__ u, double *__restrict__ part) {
+  // part[blockIdx.x * J + j] for j in this block's tile [blockIdx.y*JT, +JT)
+  const int j0 = blockIdx.y * JT;
+  double a[JT];
+#pragma unroll
+  for (int t = 0; t < JT; ++t) a[t] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    const double ui = u[i];
+#pragma unroll
+    for (int t = 0; t < JT; ++t)
+      if (j0 + t < J) a[t] += V[(int64_t)(j0 + t) * ld + i] * ui;
+  }
+  __shared__ double red[JT][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < JT; ++t) {
+    double v = a[t];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[t][w] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < JT && j0 + (int)threadIdx.x < J) {
+    const int t = threadIdx.x;
+    part[(int64_t)blockIdx.x * J + j0 + t] = ((red[t][0] + red[t][1]) + (red[t][2] + red[t][3]));
+  }
+}
+
+__global__ void k_reduce_parts(int nparts, int J, const double *__restrict__ part, double *__restrict__ h,
+                               int do_sqrt) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= J) return;
+  double s = 0.0;
+  for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * J + j];
+  h[j] = do_sqrt ? sqrt(s) : s;
+}
+
+__global__ void __launch_bounds__(256) k_gemv_update(int64_t len, int J, const double *__restrict__ V, int64_t ld,
+                                                      const double *__restrict__ h, double scale, double *__restrict__ u,
+                                                      double *__restrict__ part) {
+  extern __shared__ double hs[];
+  for (int t = threadIdx.x; t < J; t += blockDim.x) hs[t] = h[t] * scale;
+  __syncthreads();
+  double ss = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int j = 0; j < J; ++j) s += V[(int64_t)j * ld + i] * hs[j];
+    const double un = u[i] - s;
+    u[i] = un;
+    ss += un * un;
+  }
+  if (part) {
+    __shared__ double red[4];
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
+}
+
+__global__ void k_scale_div(int64_t len, const double *__restrict__ src, const double *__restrict__ div,
+                            double *__restrict__ dst, double *__restrict__ dst2) {
+  const double inv = 1.0 / div[0];
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = src[i] * inv;
+    dst[i] = v;
+    if (dst2) dst2[i] = v;
+  }
+}
+
+__global__ void k_axpy_dev(int64_t len, const double *__restrict__ a, double sign, const double *__restrict__ x,
+                           double *__restrict__ y) {
+  const double s = a[0] * sign;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += s * x[i];
+}
+
+template <typename S, typename D>
+__global__ void k_convert(int64_t len, const S *__restrict__ s, D *__restrict__ d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = (D)s[i];
+}
+
+template <int QT>
+__global__ void __launch_bounds__(256) k_rr_combine(int64_t len, int m, int k, const double *__restrict__ V, int64_t ld,
+                                                     const double *__restrict__ W, double *__restrict__ out,
+                                                     int64_t ld_out) {
+  // one thread per column i; W streamed through LDS in chunks of 64 rows
+  __shared__ double ws[64 * QT];
+  const int q0 = blockIdx.y * QT;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double a[QT];
+#pragma unroll
+  for (int q = 0; q < QT; ++q) a[q] = 0.0;
+  for (int rc = 0; rc < m; rc += 64) {
+    const int nr = (m - rc) < 64 ? (m - rc) : 64;
+    __syncthreads();
+    for (int t = threadIdx.x; t < 64 * QT; t += blockDim.x) {
+      const int r = t / QT, q = t % QT;
+      ws[t] = (r < nr && q0 + q < k) ? W[(int64_t)(rc + r) * k + q0 + q] : 0.0;
+    }
+    __syncthreads();
+    if (i < len) {
+      for (int r = 0; r < nr; ++r) {
+        const double v = V[(int64_t)(rc + r) * ld + i];
+#pragma unroll
+        for (int q = 0; q < QT; ++q) a[q] += ws[r * QT + q] * v;
+      }
+    }
+  }
+  if (i < len) {
+#pragma unroll
+    for (int q = 0; q < QT; ++q)
+      if (q0 + q < k) out[(int64_t)(q0 + q) * ld_out + i] = a[q];
+  }
+}
+
+// Triangular parts of the symmetric block matrix as the reference's half storage
+// sees it: part 0 = full A, 1 = L (scalar lower triangle incl. diagonal),
+// 2 = L^T (scalar upper triangle incl. diagonal). One thread per scalar row.
+template <typename MT, typename VT>
+__global__ void k_bsr_part(int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                           const MT *__restrict__ vals, const VT *__restrict__ x, VT *__restrict__ y, int part) {
+  const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (r >= 3 * n) return;
+  const int64_t i = r / 3;
+  const int p = (int)(r % 3);
+  double acc = 0.0;
+  for (int64_t b = rowptr[i]; b < rowptr[i + 1]; ++b) {
+    const int64_t j = col[b];
+    if (part == 1 && j > i) continue;
+    if (part == 2 && j < i) continue;
+    for (int q = 0; q < 3; ++q) {
+      if (j == i && part == 1 && q > p) continue;
+      if (j == i && part == 2 && q < p) continue;
+      acc += (double)vals[9 * b + 3 * p + q] * (double)x[3 * j + q];
+    }
+  }
+  y[r] = (VT)acc;
+}
+
+template <typename X, typename Y>
+__global__ void k_axpby(int64_t len, double a, const X *__restrict__ x, double b, Y *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    if (b == 0.0)
+      y[i] = (Y)(a * (double)x[i]);
+    else
+      y[i] = (Y)(a * (double)x[i] + b * (double)y[i]);
+  }
+}
+
+int grid_for(int64_t len, int bs, int cap) {
+  int64_t g = (len + bs - 1) / bs;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <typename MT, typename VT>
+inch_status launch_spmv(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals, const void *x,
+                        void *y, double s1, double c0, const void *z, double c2, void *acc, double mu,
+                        const double *Q, const double *qcoef, int mq, int G, hipStream_t st) {
+  const int bs = 256;
+  const int64_t grid = (n * G + bs - 1) / bs;
+#define INCH_SPMV_CASE(GG)                                                                                          \
+  case GG:                                                                                                          \
+    k_bsr_spmv<MT, VT, GG><<<grid, bs, 0, st>>>(n, rowptr, col, (const MT *)vals, (const VT *)x, (VT *)y, (MT)s1,   \
+                                                (MT)c0, (const VT *)z, (MT)c2, (VT *)acc, (MT)mu, Q, qcoef, mq); \
+    break;
+  switch (G) {
+    INCH_SPMV_CASE(4)
+    INCH_SPMV_CASE(8)
+    INCH_SPMV_CASE(16)
+    INCH_SPMV_CASE(32)
+    INCH_SPMV_CASE(64)
+    default: INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmv: lanes_per_row must be 4/8/16/32/64 (got %d)", G);
+  }
+#undef INCH_SPMV_CASE
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+template <typename MT, typename VT>
+inch_status launch_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals, const void *X,
+                        void *Y, int nrhs, hipStream_t st) {
+  constexpr int G = 8, R = 8;
+  const int bs = 256;
+  dim3 grid((unsigned)((n * G + bs - 1) / bs), (unsigned)((nrhs + R - 1) / R));
+  k_bsr_spmm<MT, VT, G, R><<<grid, bs, 0, st>>>(n, rowptr, col, (const MT *)vals, (const VT *)X, (VT *)Y, nrhs);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+inch_status inch_bsr_spmv(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals, int32_t mdt,
+                          int32_t vdt, const void *x, void *y, double s1, double c0, const void *z, double c2,
+                          void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
+                          int32_t lanes_per_row, void *stream) {
+  INCH_REQUIRE(n > 0 && rowptr && col && vals && x && y, "inch_bsr_spmv: bad args");
+  INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_bsr_spmv: qcoef needs Q and m_q > 0");
+  INCH_REQUIRE(x != y, "inch_bsr_spmv: x and y must not alias");
+  hipStream_t st = (hipStream_t)stream;
+  const int G = lanes_per_row > 0 ? lanes_per_row : 16;
+  if (mdt == INCH_F32 && vdt == INCH_F32)
+    return launch_spmv<float, float>(n, rowptr, col, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
+  if (mdt == INCH_F32 && vdt == INCH_F64)
+    return launch_spmv<float, double>(n, rowptr, col, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
+  if (mdt == INCH_F64 && vdt == INCH_F64)
+    return launch_spmv<double, double>(n, rowptr, col, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
+  INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmv: unsupported dtype pair (%d,%d)", mdt, vdt);
+}
+
+inch_status inch_bsr_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals, int32_t mdt,
+                          int32_t vdt, const void *X, void *Y, int32_t nrhs, void *stream) {
+  INCH_REQUIRE(n > 0 && rowptr && col && vals && X && Y && nrhs > 0 && X != Y, "inch_bsr_spmm: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  if (mdt == INCH_F32 && vdt == INCH_F32) return launch_spmm<float, float>(n, rowptr, col, vals, X, Y, nrhs, st);
+  if (mdt == INCH_F32 && vdt == INCH_F64) return launch_spmm<float, double>(n, rowptr, col, vals, X, Y, nrhs, st);
+  if (mdt == INCH_F64 && vdt == INCH_F64) return launch_spmm<double, double>(n, rowptr, col, vals, X, Y, nrhs, st);
+  INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmm: unsupported dtype pair (%d,%d)", mdt, vdt);
+}
+
+inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                   double plus_i, const double *x, double *y, int32_t nrhs, void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && rowptr && col && x && y && nrhs > 0 && x != y, "inch_anm_matfree_apply: bad args");
+  INCH_REQUIRE(nrhs <= 65535, "inch_anm_matfree_apply: nrhs too large");
+  hipStream_t st = (hipStream_t)stream;
+  constexpr int G = 16;
+  const int bs = 256;
+  dim3 grid((unsigned)((n * G + bs - 1) / bs), (unsigned)nrhs);
+  k_matfree<G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, x, y, nrhs);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+int64_t inch_multidot_work_size(int32_t J) { return (int64_t)kRedBlocks * (J > 0 ? J : 1); }
+
+inch_status inch_multidot(int64_t len, int32_t J, const double *V, int64_t ld, const double *u, double *h,
+                          double *work, void *stream) {
+  INCH_REQUIRE(len > 0 && J > 0 && V && u && h && work && ld >= len, "inch_multidot: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  constexpr int JT = 8;
+  const int gx = grid_for(len, 256, kRedBlocks);
+  dim3 grid(gx, (J + JT - 1) / JT);
+  k_multidot_part<JT><<<grid, 256, 0, st>>>(len, J, V, ld, u, work);
+  HIP_TRY(hipGetLastError());
+  k_reduce_parts<<<(J + 255) / 256, 256, 0, st>>>(gx, J, work, h, 0);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_gemv_update(int64_t len, int32_t J, const double *V, int64_t ld, const double *h, double scale,
+                             double *u, double *nrm2_out, double *work, void *stream) {
+  INCH_REQUIRE(len > 0 && J >= 0 && u && (J == 0 || (V && h)) && ld >= len, "inch_gemv_update: bad args");
+  INCH_REQUIRE(!nrm2_out || work, "inch_gemv_update: nrm2 needs work");
+  INCH_REQUIRE(J <= 8192, "inch_gemv_update: J too large");
+  hipStream_t st = (hipStream_t)stream;
+  const int gx = grid_for(len, 256, kRedBlocks);
+  k_gemv_update<<<gx, 256, (size_t)(J > 0 ? J : 1) * sizeof(double), st>>>(len, J, V, ld, h, scale, u,
+                                                                           nrm2_out ? work : nullptr);
+  HIP_TRY(hipGetLastError());
+  if (nrm2_out) {
+    k_reduce_parts<<<1, 256, 0, st>>>(gx, 1, work, nrm2_out, 1);
+    HIP_TRY(hipGetLastError());
+  }
+  return INCH_OK;
+}
+
+inch_status inch_vec_scale_div(int64_t len, const double *src, const double *div, double *dst, double *dst2,
+                               void *stream) {
+  INCH_REQUIRE(len > 0 && src && div && dst, "inch_vec_scale_div: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  k_scale_div<<<grid_for(len, 256, 4096), 256, 0, st>>>(len, src, div, dst, dst2);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_vec_axpy_dev(int64_t len, const double *a, double sign, const double *x, double *y, void *stream) {
+  INCH_REQUIRE(len > 0 && a && x && y, "inch_vec_axpy_dev: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  k_axpy_dev<<<grid_for(len, 256, 4096), 256, 0, st>>>(len, a, sign, x, y);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_vec_convert(int64_t len, const void *src, int32_t sdt, void *dst, int32_t ddt, void *stream) {
+  INCH_REQUIRE(len > 0 && src && dst, "inch_vec_convert: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  const int g = grid_for(len, 256, 4096);
+  if (sdt == INCH_F64 && ddt == INCH_F32)
+    k_convert<double, float><<<g, 256, 0, st>>>(len, (const double *)src, (float *)dst);
+  else if (sdt == INCH_F32 && ddt == INCH_F64)
+    k_convert<float, double><<<g, 256, 0, st>>>(len, (const float *)src, (double *)dst);
+  else if (sdt == INCH_F64 && ddt == INCH_F64)
+    k_convert<double, double><<<g, 256, 0, st>>>(len, (const double *)src, (double *)dst);
+  else if (sdt == INCH_F32 && ddt == INCH_F32)
+    k_convert<float, float><<<g, 256, 0, st>>>(len, (const float *)src, (float *)dst);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_vec_convert: bad dtypes");
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_bsr_spmv_part(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals,
+                               int32_t mdt, int32_t vdt, const void *x, void *y, int32_t part, void *stream) {
+  INCH_REQUIRE(n > 0 && rowptr && col && vals && x && y && x != y && part >= 0 && part <= 2,
+               "inch_bsr_spmv_part: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t g = (3 * n + 255) / 256;
+  if (mdt == INCH_F64 && vdt == INCH_F64)
+    k_bsr_part<double, double><<<g, 256, 0, st>>>(n, rowptr, col, (const double *)vals, (const double *)x,
+                                                  (double *)y, part);
+  else if (mdt == INCH_F32 && vdt == INCH_F32)
+    k_bsr_part<float, float><<<g, 256, 0, st>>>(n, rowptr, col, (const float *)vals, (const float *)x, (float *)y,
+                                                part);
+  else if (mdt == INCH_F32 && vdt == INCH_F64)
+    k_bsr_part<float, double><<<g, 256, 0, st>>>(n, rowptr, col, (const float *)vals, (const double *)x,
+                                                 (double *)y, part);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmv_part: unsupported dtype pair");
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_vec_axpby(int64_t len, double a, const void *x, int32_t xdt, double b, void *y, int32_t ydt,
+                           void *stream) {
+  INCH_REQUIRE(len > 0 && x && y, "inch_vec_axpby: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  const int g = grid_for(len, 256, 4096);
+  if (xdt == INCH_F64 && ydt == INCH_F64)
+    k_axpby<double, double><<<g, 256, 0, st>>>(len, a, (const double *)x, b, (double *)y);
+  else if (xdt == INCH_F32 && ydt == INCH_F32)
+    k_axpby<float, float><<<g, 256, 0, st>>>(len, a, (const float *)x, b, (float *)y);
+  else if (xdt == INCH_F64 && ydt == INCH_F32)
+    k_axpby<double, float><<<g, 256, 0, st>>>(len, a, (const double *)x, b, (float *)y);
+  else if (xdt == INCH_F32 && ydt == INCH_F64)
+    k_axpby<float, double><<<g, 256, 0, st>>>(len, a, (const float *)x, b, (double *)y);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_vec_axpby: bad dtypes");
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, int64_t ld, const double *W,
+                            double *out, int64_t ld_out, void *stream) {
+  INCH_REQUIRE(len > 0 && m > 0 && k > 0 && V && W && out && ld >= len && ld_out >= len, "inch_rr_combine: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t gx = (len + 255) / 256;
+  INCH_REQUIRE(gx < ((int64_t)1 << 31), "inch_rr_combine: len too large");
+  if (k > 16) {
+    constexpr int QT = 32;
+    dim3 grid((unsigned)gx, (k + QT - 1) / QT);
+    k_rr_combine<QT><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
+  } else {
+    constexpr int QT = 16;
+    dim3 grid((unsigned)gx, (k + QT - 1) / QT);
+    k_rr_combine<QT><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
+  }
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,221 @@
+"""ANM Hessian on the device: construction pipeline and the block-row matrix container.
+
+Pipeline (replaces ``X_KdCuthillMckeeOrder`` + ``X_KdUngappedMinMaxNeighbor`` +
+``Xnumpy_SparseCupyMatrixUngappped.ReturnCupyHLowerTriangleInt32`` of the reference,
+``InchingLiteInteger/Fuel/Coordinate/T1.py:171-481`` and ``InchingLiteInteger/Fuel/T1.py:67-660``):
+
+1. cell-list neighbour CSR of the input coordinates (device kernel);
+2. reverse Cuthill-McKee order (host C++, bit-exact to the reference's 3DRCM with a
+   pinned stable tie rule, or any supplied seed order);
+3. coordinates permuted on the device, neighbour CSR recomputed in the new order;
+4. 3x3 super-element assembly straight into full-storage block rows, in float64 and/or
+   float32 (the float32 copy is the round-to-nearest of the float64 values).
+
+The container ``BlockHessian`` keeps the coordinates and the neighbour pattern, so the
+matrix-free FP64 operator (residual polish) needs no float64 copy of the values.
+"""
+import numpy as np
+import torch
+
+from . import ops
+
+
+class BlockHessian:
+    """A = H + PlusI * I, full-storage 3x3 block rows in RCM order on the GPU.
+
+    Quacks like the reference's lower-triangle CSR where callers rely on it:
+    ``A @ v`` is the scalar lower triangle L v, ``A.T @ v`` is L^T v and
+    ``A.diagonal(0)`` is A_diag, so ``A @ v + A.T @ v - A_diag * v == full A v`` as in
+    ``Burn/Krylov/T3.py:131-160``. ``A.matvec(v)`` is the fused full product.
+    """
+
+    def __init__(self, xyz, rowptr, col, vals64, vals32, diag, plus_i, rc, order=None, undo=None):
+        self.xyz = xyz
+        self.rowptr = rowptr
+        self.col = col
+        self.vals64 = vals64
+        self.vals32 = vals32
+        self.A_diag = diag
+        self.plus_i = float(plus_i)
+        self.rc = float(rc)
+        self.order = order
+        self.undo = undo
+        self.n_atoms = int(xyz.shape[0])
+        self.shape = (3 * self.n_atoms, 3 * self.n_atoms)
+        self.ndim = 2
+        self.lanes = 16
+        self._transposed = False
+
+    # ----------------------------------------------------------- properties --
+    @property
+    def dtype(self):
+        return np.dtype(np.float64) if self.vals64 is not None else np.dtype(np.float32)
+
+    @property
+    def device(self):
+        return self.xyz.device
+
+    @property
+    def nnzb(self):
+        return int(self.col.numel())
+
+    def vals(self, precision):
+        v = self.vals64 if precision == "f64" else self.vals32
+        if v is None:
+            raise ValueError("this matrix holds no %s values" % precision)
+        return v
+
+    def ensure(self, precision):
+        """Make sure the values of ``precision`` exist (assembled on demand)."""
+        if (precision == "f64" and self.vals64 is None) or (precision == "f32" and self.vals32 is None):
+            v64, v32, _ = ops.hessian_assemble(self.xyz, self.rc, self.plus_i, self.rowptr, self.col,
+                                               want64=precision == "f64", want32=precision == "f32")
+            if precision == "f64":
+                self.vals64 = v64
+            else:
+                self.vals32 = v32
+
+    # ------------------------------------------------------------ products --
+    def matvec(self, x, y=None, precision=None, **epi):
+        """y = A x (+ fused epilogue, see ops.bsr_spmv); precision 'f64' / 'f32'."""
+        if precision is None:
+            precision = "f64" if self.vals64 is not None else "f32"
+        if y is None:
+            y = torch.empty_like(x)
+        ops.bsr_spmv(self.n_atoms, self.rowptr, self.col, self.vals(precision), x, y, lanes=self.lanes, **epi)
+        return y
+
+    def matmat(self, X, precision=None):
+        if precision is None:
+            precision = "f64" if self.vals64 is not None else "f32"
+        Y = torch.empty_like(X)
+        ops.bsr_spmm(self.n_atoms, self.rowptr, self.col, self.vals(precision), X.contiguous(), Y)
+        return Y
+
+    def matfree(self, X):
+        """FP64 (H + PlusI I) X recomputed from coordinates (X [3n] or [3n, r])."""
+        Y = torch.empty_like(X)
+        ops.matfree_apply(self.xyz, self.rowptr, self.col, self.plus_i, X.contiguous(), Y)
+        return Y
+
+    def _part(self, v, part):
+        prec = "f64" if self.vals64 is not None else "f32"
+        vv = v.reshape(-1).contiguous()
+        if prec == "f64" and vv.dtype != torch.float64:
+            vv = vv.double()
+        y = torch.empty_like(vv)
+        ops.bsr_spmv_part(self.n_atoms, self.rowptr, self.col, self.vals(prec), vv, y, part)
+        return y.reshape(v.shape)
+
+    def __matmul__(self, v):
+        if v.dim() == 2 and v.shape[1] != 1:
+            return torch.stack([self.__matmul__(v[:, r]) for r in range(v.shape[1])], dim=1)
+        return self._part(v, 2 if self._transposed else 1)
+
+    @property
+    def T(self):
+        t = BlockHessian.__new__(BlockHessian)
+        t.__dict__.update(self.__dict__)
+        t._transposed = not self._transposed
+        return t
+
+    def diagonal(self, k=0):
+        if k != 0:
+            raise NotImplementedError("only the main diagonal is available")
+        return self.A_diag
+
+    # ------------------------------------------------------- reference CSR --
+    def to_lower_csr(self):
+        """Scalar CSR (indptr, indices, data) of tril(A) in float64 numpy arrays (host)."""
+        import scipy.sparse as sp
+        self.ensure("f64")
+        rp = self.rowptr.cpu().numpy()
+        col = self.col.cpu().numpy().astype(np.int64)
+        vals = self.vals64.cpu().numpy().reshape(-1, 3, 3)
+        B = sp.bsr_matrix((vals, col, rp), shape=self.shape).tocsr()
+        L = sp.tril(B, k=0, format="csr")
+        L.eliminate_zeros()
+        L.sort_indices()
+        return L.indptr.astype(np.int64), L.indices.astype(np.int64), L.data
+
+
+def _to_device_xyz(X, device):
+    if isinstance(X, torch.Tensor):
+        t = X.detach().to(device=device, dtype=torch.float64)
+    else:
+        t = torch.as_tensor(np.ascontiguousarray(X, dtype=np.float64), device=device)
+    if t.dim() != 2 or t.shape[1] != 3:
+        raise ValueError("coordinates must be [n_atoms, 3]")
+    return t.contiguous()
+
+
+def rcm_order(X, rc, seed_order=None, reverse=True, device=None):
+    """3DRCM permutation of coordinates X: (order int32, undo int64) numpy arrays."""
+    ops.require_gpu()
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    xyz = _to_device_xyz(X, device)
+    indptr, indices = ops.neighbours(xyz, rc)
+    return ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order, reverse=reverse)
+
+
+def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_order=None, device=None):
+    """Coordinates -> BlockHessian (in RCM order when ``reorder``).
+
+    ``precision``: which value copies to assemble ('f32', 'f64'). Returns the container;
+    ``A.order`` / ``A.undo`` hold the permutation (identity when not reordered).
+    """
+    ops.require_gpu()
+    device = device or torch.device("cuda", torch.cuda.current_device())
+    xyz = _to_device_xyz(X, device)
+    n = xyz.shape[0]
+    if reorder:
+        indptr, indices = ops.neighbours(xyz, rc)
+        order, undo = ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order)
+        del indptr, indices
+        xyz = xyz[torch.as_tensor(order.astype(np.int64), device=device)].contiguous()
+    else:
+        order = np.arange(n, dtype=np.int32)
+        undo = np.arange(n, dtype=np.int64)
+    rowptr, col = ops.neighbours(xyz, rc)
+    v64, v32, diag = ops.hessian_assemble(xyz, rc, plus_i, rowptr, col, want64="f64" in precision,
+                                          want32="f32" in precision)
+    return BlockHessian(xyz, rowptr, col, v64, v32, diag, plus_i, rc, order=order, undo=undo)
+
+
+def rigid_body_basis(xyz):
+    """6 x 3N rigid-body basis of device coordinates, built as the reference driver builds it
+    (Command8A/commandBenchmark_InchingTRLMHD_Linux_Pdb_0064_A100.py:314-372): three unit
+    translations (1/sqrt N), three infinitesimal rotations X @ R_a normalised, then two sweeps
+    of double Gram-Schmidt. float64 torch [6, 3N] on the device of ``xyz``."""
+    x = xyz.to(torch.float64)
+    n = x.shape[0]
+    Q = torch.zeros((6, 3 * n), dtype=torch.float64, device=x.device)
+    for a in range(3):
+        Q[a].view(n, 3)[:, a] = 1.0 / np.sqrt(n)
+    # e_a x p for the three axes
+    rots = [torch.stack([torch.zeros_like(x[:, 0]), -x[:, 2], x[:, 1]], 1),
+            torch.stack([x[:, 2], torch.zeros_like(x[:, 0]), -x[:, 0]], 1),
+            torch.stack([-x[:, 1], x[:, 0], torch.zeros_like(x[:, 0])], 1)]
+    for a in range(3):
+        g = rots[a].reshape(-1)
+        Q[3 + a] = g / torch.linalg.norm(g)
+    for _ in range(2):
+        for ix in range(1, 6):
+            for _rep in range(2):
+                Q[ix] -= Q[:ix].T @ (Q[:ix] @ Q[ix])
+                Q[ix] /= torch.sqrt(Q[ix] @ Q[ix])
+    return Q
+
+
+def component_rigid_basis(xyz, comp):
+    """Rigid-body basis per connected component (6 rows per component)."""
+    comp = torch.as_tensor(comp, device=xyz.device)
+    rows = []
+    for c in torch.unique(comp).tolist():
+        idx = torch.nonzero(comp == c).flatten()
+        Qc = rigid_body_basis(xyz[idx])
+        full = torch.zeros((6, 3 * xyz.shape[0]), dtype=torch.float64, device=xyz.device)
+        dof = (3 * idx[:, None] + torch.arange(3, device=xyz.device)[None, :]).reshape(-1)
+        full[:, dof] = Qc
+        rows.append(full)
+    return torch.cat(rows, 0)

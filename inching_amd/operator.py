@@ -1,0 +1,82 @@
+"""Device operator for the eigensolvers: u = op(v) with op one of
+
+* A v                                   (``Burn/Krylov/T3.py:54-239``)
+* p(A) v, p the Chebyshev filter        (``Burn/PolynomialFilters/T2.py:28-92``)
+* either of the above + shift Q^T Q v   (Hotelling / EED, ``Burn/Orthogonalization/T3.py:62-164``)
+
+Precision policy: the block values and the Chebyshev recurrence vectors are float32 with
+float32 accumulation in ``precision='f32'`` (mixed) mode and float64 in ``'f64'`` mode;
+v and u (the Lanczos basis side) are always float64. One filtered matvec of degree m is m
+launches of the fused SpMV + three-term-recurrence kernel.
+"""
+import torch
+
+from . import ops
+
+
+class Operator:
+    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None):
+        if precision not in ("f32", "f64"):
+            raise ValueError("precision must be 'f32' or 'f64'")
+        A.ensure(precision)
+        self.A = A
+        self.precision = precision
+        self.vals = A.vals(precision)
+        self.n = A.shape[0]
+        self.pol = pol
+        self.Q = Q
+        self.shift = float(shift)
+        self.lanes = lanes or A.lanes
+        self.n_spmv = 0
+        dev = A.device
+        vdt = torch.float32 if precision == "f32" else torch.float64
+        self.vdt = vdt
+        if pol is not None:
+            self._vk = torch.empty(self.n, dtype=vdt, device=dev)
+            self._vkm1 = torch.empty(self.n, dtype=vdt, device=dev)
+            self._vkp1 = torch.empty(self.n, dtype=vdt, device=dev)
+            self._acc = torch.empty(self.n, dtype=vdt, device=dev)
+            self.mu = [float(x) for x in pol.AdjustedCoefficients[:pol.AdjustedDegree + 1]]
+            self.degree = int(pol.AdjustedDegree)
+        if Q is not None:
+            if Q.dtype != torch.float64 or Q.shape[1] != self.n:
+                raise ValueError("Q must be float64 [m, 3N]")
+            self.Q = Q.contiguous()
+            self._h = torch.empty(Q.shape[0], dtype=torch.float64, device=dev)
+            self._ws = ops.Workspace(Q.shape[0], dev)
+
+    def spmv(self, x, y, **epi):
+        ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.lanes, **epi)
+        self.n_spmv += 1
+
+    def filtered(self, v64, u64):
+        """u = sum_k mu_k T_k((A - cc)/dd) v."""
+        pol = self.pol
+        dd, cc = float(pol.dd), float(pol.cc)
+        vk, vkm1, vkp1, acc = self._vk, self._vkm1, self._vkp1, self._acc
+        ops.convert(v64, vk)
+        ops.axpby(self.mu[0], v64, 0.0, acc)
+        for k in range(1, self.degree + 1):
+            if k == 1:
+                self.spmv(vk, vkp1, s1=1.0 / dd, c0=cc, acc=acc, mu=self.mu[1])
+            else:
+                self.spmv(vk, vkp1, s1=2.0 / dd, c0=cc, z=vkm1, c2=1.0, acc=acc, mu=self.mu[k])
+            vkm1, vk, vkp1 = vk, vkp1, vkm1
+        self._vk, self._vkm1, self._vkp1 = vk, vkm1, vkp1
+        ops.convert(acc, u64)
+
+    def apply(self, v64, u64):
+        if self.pol is None:
+            self.spmv(v64, u64)
+        else:
+            self.filtered(v64, u64)
+        if self.Q is not None and self.shift != 0.0:
+            m = self.Q.shape[0]
+            ops.multidot(self.Q, m, v64, self._h, self._ws)
+            ops.gemv_update(self.Q, m, self._h, u64, scale=-self.shift)
+        return u64
+
+    def plain(self, v64, u64):
+        """A v without filter and without the Hotelling term."""
+        self.spmv(v64, u64)
+        return u64

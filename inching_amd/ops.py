@@ -1,0 +1,222 @@
+"""Typed wrappers of the C ABI on torch tensors (device memory + the current HIP stream).
+
+Every wrapper checks shapes and dtypes on the host before launching, so a kernel never
+sees a buffer smaller than its grid assumes.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import F32, F64, call
+
+_DT = {torch.float32: F32, torch.float64: F64}
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("inching_amd: a ROCm GPU is required for device kernels (torch.cuda.is_available() is False)")
+
+
+def stream_ptr(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def P(t):
+    """Raw device/host pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def dt(t):
+    try:
+        return _DT[t.dtype]
+    except KeyError:
+        raise TypeError("unsupported dtype %s" % t.dtype)
+
+
+def _check_dev(*ts):
+    for t in ts:
+        if t is not None:
+            if not t.is_cuda:
+                raise ValueError("tensor must live on the GPU")
+            if not t.is_contiguous():
+                raise ValueError("tensor must be contiguous")
+
+
+# ------------------------------------------------------------------ host ----
+
+def synth_globule(n, radius, dmin, seed, max_attempts=0):
+    out = np.empty((int(n), 3), dtype=np.float64)
+    call("inch_synth_globule", int(n), float(radius), float(dmin), int(seed) & 0xFFFFFFFFFFFFFFFF,
+         int(max_attempts), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def rcm_host(indptr, indices, seed_order=None, reverse=True):
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    n = indptr.shape[0] - 1
+    order = np.empty(n, dtype=np.int32)
+    undo = np.empty(n, dtype=np.int64)
+    so = None
+    if seed_order is not None:
+        so = np.ascontiguousarray(seed_order, dtype=np.int64)
+        if so.shape != (n,):
+            raise ValueError("seed_order must have shape (n,)")
+    call("inch_rcm_host", n, indptr.ctypes.data_as(C.c_void_p), indices.ctypes.data_as(C.c_void_p),
+         so.ctypes.data_as(C.c_void_p) if so is not None else None, 1 if reverse else 0,
+         order.ctypes.data_as(C.c_void_p), undo.ctypes.data_as(C.c_void_p))
+    return order, undo
+
+
+def batch_head(n, maxleaf):
+    cap = 2 * (int(n) // max(1, int(maxleaf) // 2) + 2) + 8
+    head = np.empty(cap, dtype=np.int64)
+    nb = C.c_int64(0)
+    call("inch_batch_head", int(n), int(maxleaf), head.ctypes.data_as(C.c_void_p), cap, C.byref(nb))
+    return head[:nb.value + 1].copy()
+
+
+# ---------------------------------------------------------------- device ----
+
+def neighbours(xyz, rc, stream=None):
+    """Cell-list neighbour CSR of device float64 coordinates [n,3] -> (indptr, indices)."""
+    require_gpu()
+    _check_dev(xyz)
+    if xyz.dtype != torch.float64 or xyz.dim() != 2 or xyz.shape[1] != 3:
+        raise ValueError("xyz must be float64 [n,3]")
+    n = xyz.shape[0]
+    indptr = torch.empty(n + 1, dtype=torch.int64, device=xyz.device)
+    nnz = C.c_int64(0)
+    call("inch_neighbors_celllist", n, P(xyz), float(rc), P(indptr), None, 0, C.byref(nnz), stream_ptr(stream))
+    indices = torch.empty(max(1, nnz.value), dtype=torch.int32, device=xyz.device)
+    call("inch_neighbors_celllist", n, P(xyz), float(rc), P(indptr), P(indices), nnz.value, C.byref(nnz),
+         stream_ptr(stream))
+    return indptr, indices[:nnz.value]
+
+
+def hessian_assemble(xyz, rc, plus_i, rowptr, col, want64=True, want32=True, stream=None):
+    require_gpu()
+    _check_dev(xyz, rowptr, col)
+    n = xyz.shape[0]
+    nnzb = col.shape[0]
+    if rowptr.shape[0] != n + 1:
+        raise ValueError("rowptr must have n+1 entries")
+    v64 = torch.empty(nnzb * 9, dtype=torch.float64, device=xyz.device) if want64 else None
+    v32 = torch.empty(nnzb * 9, dtype=torch.float32, device=xyz.device) if want32 else None
+    diag = torch.empty(3 * n, dtype=torch.float64, device=xyz.device)
+    call("inch_hessian_assemble", n, P(xyz), float(rc), float(plus_i), P(rowptr), P(col), P(v64), P(v32),
+         P(diag), stream_ptr(stream))
+    return v64, v32, diag
+
+
+def bsr_spmv(n_atoms, rowptr, col, vals, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None,
+             qcoef=None, lanes=16, stream=None):
+    nd = 3 * n_atoms
+    for t in (x, y, z, acc):
+        if t is not None and t.numel() != nd:
+            raise ValueError("vector length mismatch")
+    if vals.numel() != 9 * col.numel() or rowptr.numel() != n_atoms + 1:
+        raise ValueError("matrix arrays inconsistent")
+    vd = dt(x)
+    for t in (y, z, acc):
+        if t is not None and dt(t) != vd:
+            raise ValueError("all vectors must share a dtype")
+    mq = 0
+    if qcoef is not None:
+        if Q is None or Q.dtype != torch.float64 or Q.shape[1] != nd:
+            raise ValueError("Q must be float64 [m, 3n]")
+        mq = Q.shape[0]
+        if qcoef.numel() < mq:
+            raise ValueError("qcoef too short")
+    call("inch_bsr_spmv", n_atoms, P(rowptr), P(col), P(vals), dt(vals), vd, P(x), P(y), float(s1), float(c0),
+         P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(lanes), stream_ptr(stream))
+
+
+def bsr_spmv_part(n_atoms, rowptr, col, vals, x, y, part, stream=None):
+    """part 0: y = A x, 1: y = L x, 2: y = L^T x (reference half-storage views)."""
+    if x.numel() != 3 * n_atoms or y.numel() != 3 * n_atoms:
+        raise ValueError("vector length mismatch")
+    call("inch_bsr_spmv_part", n_atoms, P(rowptr), P(col), P(vals), dt(vals), dt(x), P(x), P(y), int(part),
+         stream_ptr(stream))
+
+
+def bsr_spmm(n_atoms, rowptr, col, vals, X, Y, stream=None):
+    nd = 3 * n_atoms
+    if X.shape != Y.shape or X.shape[0] != nd or X.dim() != 2:
+        raise ValueError("X, Y must be [3n, nrhs]")
+    call("inch_bsr_spmm", n_atoms, P(rowptr), P(col), P(vals), dt(vals), dt(X), P(X), P(Y), X.shape[1],
+         stream_ptr(stream))
+
+
+def matfree_apply(xyz, rowptr, col, plus_i, X, Y, stream=None):
+    n = xyz.shape[0]
+    if X.dim() == 1:
+        X2, Y2 = X.view(-1, 1), Y.view(-1, 1)
+    else:
+        X2, Y2 = X, Y
+    if X2.shape[0] != 3 * n or X2.shape != Y2.shape or X2.dtype != torch.float64 or Y2.dtype != torch.float64:
+        raise ValueError("X, Y must be float64 [3n, nrhs]")
+    call("inch_anm_matfree_apply", n, P(xyz), P(rowptr), P(col), float(plus_i), P(X2), P(Y2), X2.shape[1],
+         stream_ptr(stream))
+
+
+class Workspace:
+    """Scratch for the two-phase reductions, sized for up to ``J`` dot products."""
+
+    def __init__(self, J, device):
+        self.J = max(1, int(J))
+        from ._lib import lib
+        self.buf = torch.empty(int(lib().inch_multidot_work_size(self.J)), dtype=torch.float64, device=device)
+
+
+def multidot(V, J, u, h, ws, stream=None):
+    """h[:J] = V[:J] @ u  (V [rows, len] float64 row-major)."""
+    if J > ws.J:
+        raise ValueError("workspace too small")
+    if V.shape[0] < J or V.shape[1] != u.numel() or h.numel() < J:
+        raise ValueError("shape mismatch in multidot")
+    call("inch_multidot", u.numel(), int(J), P(V), V.stride(0), P(u), P(h), P(ws.buf), stream_ptr(stream))
+
+
+def gemv_update(V, J, h, u, scale=1.0, nrm2_out=None, ws=None, stream=None):
+    """u -= scale * V[:J]^T h[:J]; optionally nrm2_out[0] = ||u||."""
+    if J > 0 and (V.shape[0] < J or V.shape[1] != u.numel() or h.numel() < J):
+        raise ValueError("shape mismatch in gemv_update")
+    call("inch_gemv_update", u.numel(), int(J), P(V) if J > 0 else None, V.stride(0) if J > 0 else u.numel(),
+         P(h) if J > 0 else None, float(scale), P(u), P(nrm2_out), P(ws.buf) if ws is not None else None,
+         stream_ptr(stream))
+
+
+def scale_div(src, div, dst, dst2=None, stream=None):
+    call("inch_vec_scale_div", src.numel(), P(src), P(div), P(dst), P(dst2), stream_ptr(stream))
+
+
+def axpy_dev(a, sign, x, y, stream=None):
+    call("inch_vec_axpy_dev", x.numel(), P(a), float(sign), P(x), P(y), stream_ptr(stream))
+
+
+def axpby(a, x, b, y, stream=None):
+    if x.numel() != y.numel():
+        raise ValueError("length mismatch")
+    call("inch_vec_axpby", x.numel(), float(a), P(x), dt(x), float(b), P(y), dt(y), stream_ptr(stream))
+
+
+def convert(src, dst, stream=None):
+    if src.numel() != dst.numel():
+        raise ValueError("length mismatch")
+    call("inch_vec_convert", src.numel(), P(src), dt(src), P(dst), dt(dst), stream_ptr(stream))
+
+
+def rr_combine(V, m, W, out, stream=None):
+    """out[:k] = W^T V[:m]   (W float64 [m, k] on device, out [>=k, len])."""
+    k = W.shape[1]
+    if W.shape[0] != m or V.shape[0] < m or out.shape[0] < k or out.shape[1] != V.shape[1]:
+        raise ValueError("shape mismatch in rr_combine")
+    if out.data_ptr() == V.data_ptr():
+        raise ValueError("rr_combine output must not alias V")
+    call("inch_rr_combine", V.shape[1], int(m), int(k), P(V), V.stride(0), P(W.contiguous()), P(out),
+         out.stride(0), stream_ptr(stream))

@@ -1,0 +1,132 @@
+"""End-to-end lowest-mode solver: coordinates -> k non-trivial modes at a relative residual.
+
+Protocol (the reference's CTRLM workflow, ``GoogleColab/GoogleColab_Inching_v023_ReleaseOkay.ipynb``
+cell 9 and ``Notebook/Application/04_Inching_EigenproblemManyCTRLM.ipynb`` cell 5):
+spectrum bound -> Chebyshev filter for the low end [PlusI, a] -> TRLM on p(A) with the
+rigid-body block deflated by a Hotelling shift -> Rayleigh quotients with A.
+
+Precision modes:
+* ``"f64"``   float64 block values throughout (the plain FP64 GPU solver);
+* ``"mixed"`` phase 1 runs the float32 operator (values and Chebyshev vectors float32,
+              float64 basis / re-orthogonalisation / Rayleigh-Ritz); phase 2 restarts
+              TRLM with the float64 operator from a combination of the phase-1 Ritz
+              vectors (an excellent start vector, so few float64 restarts are needed);
+              a float32 operator alone stalls at ||E|| ~ 6e-8 ||A|| relative residual.
+Acceptance is always the float64 residual ``||A v - lam v|| / (lam_max ||v||)`` computed
+with the matrix-free float64 operator, after a float64 Rayleigh-Ritz polish.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import chebyshev as CH
+from .operator import Operator
+from .polish import apply_fp64, orthonormalise, polish, residual_norms
+from .trlm import TRLM
+
+
+class ModeResult:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+    def as_dict(self):
+        return {k: v for k, v in self.__dict__.items() if not isinstance(v, torch.Tensor)}
+
+
+def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
+    """TRLM callback: float64 RR polish of the current Ritz block; stop when all k pairs
+    meet ``tol_rel`` or (``stall`` given) when the worst residual improves by less than
+    that factor between restarts."""
+    state = {"calls": 0, "last": None, "hist": []}
+
+    def check(trlm, S):
+        state["calls"] += 1
+        theta, X, info = polish(A, trlm.V[:k], Q=Q, lam_max=lam_max)
+        rel = info["res"] / lam_max
+        worst = float(np.max(rel))
+        state["last"] = (theta, X, rel)
+        state["hist"].append(worst)
+        if worst <= tol_rel:
+            return True
+        h = state["hist"]
+        if stall is not None and len(h) >= 2 and h[-1] > stall * h[-2]:
+            return True
+        return False
+
+    return check, state
+
+
+def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, bound=None, wanted_right=None,
+                 max_degree=5120, min_degree=5, ratio=0.7, hotelling=None, factor=2, maxiter=200, seed=0,
+                 phase1_rel=3e-7, v0=None, verbose=False):
+    """k lowest non-trivial eigenpairs of A = H + PlusI I (RCM order) on the device.
+
+    Returns ModeResult with eigval (ascending, includes PlusI), eigvec rows [k, n] (float64
+    device), rel_res, per-phase seconds and SpMV counts."""
+    t_all = time.time()
+    phases = {}
+    n_spmv = {}
+    torch.cuda.synchronize()
+    t = time.time()
+    if bound is None:
+        lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else "f64")
+        bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
+        n_spmv["bound"] = 301
+    lam_max = bound[1]
+    phases["bound"] = time.time() - t
+    pol = None
+    if filtered:
+        t = time.time()
+        if wanted_right is None:
+            # KPM bisection for an interval holding about 3 (k + rigid) eigenvalues, as in the
+            # reference's CTRLM set-up (WantedNumber = (k - 5) * 3, margin 25, 5 probes)
+            pol, cnt, interval, nk = CH.optimize_on_memory(
+                A, bound, A.plus_i, wanted_count=max(3 * k, k + 32), designated_start=bound[1] / 10.0,
+                max_degree=max_degree, min_degree=min_degree, margin=max(5, k // 2), trials=5, ratio=ratio,
+                seed=seed, precision="f32" if precision == "mixed" else "f64")
+            n_spmv["kpm"] = nk
+        else:
+            pol = CH.design_filter(A.plus_i, wanted_right, bound, max_degree, min_degree, ratio)
+        phases["filter"] = time.time() - t
+    if hotelling is None:
+        hotelling = -40.0 if filtered else 10.0
+    check, state = _check_factory(A, Q, lam_max, tol_rel, k)
+    start = v0
+    total = 0
+    if precision == "mixed":
+        t = time.time()
+        op32 = Operator(A, "f32", pol=pol, Q=Q, shift=hotelling)
+        # phase 1: float32 operator until the float64-verified residual reaches phase1_rel
+        # or stops improving (the float32 floor)
+        check1, state1 = _check_factory(A, Q, lam_max, phase1_rel, k, stall=0.7)
+        r1 = TRLM(op32, k, factor=factor).run(maxiter=maxiter, tol=0.0, v0=start, seed=seed, filtered=filtered,
+                                                check=check1, verbose=verbose)
+        phases["phase1_f32"] = time.time() - t
+        n_spmv["phase1_f32"] = op32.n_spmv
+        total += op32.n_spmv
+        theta1, X1, rel1 = state1["last"]
+        if float(np.max(rel1)) <= tol_rel:
+            torch.cuda.synchronize()
+            return ModeResult(eigval=np.asarray(theta1), eigvec=X1, rel_res=np.asarray(rel1), lam_max=lam_max,
+                              phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r1.iters + 1,
+                              degree=int(pol.AdjustedDegree) if pol is not None else 0,
+                              seconds=time.time() - t_all, precision=precision)
+        g = torch.Generator(device=A.device)
+        g.manual_seed(seed + 1)
+        c = torch.rand(k, generator=g, dtype=torch.float64, device=A.device) + 0.5
+        start = (c[:, None] * X1).sum(0)
+        del op32, r1
+    t = time.time()
+    op64 = Operator(A, "f64", pol=pol, Q=Q, shift=hotelling)
+    r2 = TRLM(op64, k, factor=factor).run(maxiter=maxiter, tol=0.0, v0=start, seed=seed, filtered=filtered,
+                                           check=check, verbose=verbose)
+    phases["phase2_f64" if precision == "mixed" else "solve_f64"] = time.time() - t
+    n_spmv["phase2_f64" if precision == "mixed" else "solve_f64"] = op64.n_spmv
+    total += op64.n_spmv
+    theta, X, rel = state["last"]
+    torch.cuda.synchronize()
+    return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(rel), lam_max=lam_max,
+                      phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r2.iters + 1,
+                      degree=int(pol.AdjustedDegree) if pol is not None else 0,
+                      seconds=time.time() - t_all, precision=precision)

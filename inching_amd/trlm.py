@@ -1,0 +1,148 @@
+"""Thick-restart Lanczos with full CGS2 re-orthogonalisation on the device.
+
+Follows ``Burn/ThickRestartLanczosHotellingDeflation/T1.py``:
+``S_HeigvalTRLMHD_HeigvecTRLMHD`` :60-310 (driver and thick restart),
+``OOC4_alpha_beta_betak_k_TrdEigvalSel_TrdEigvecSel`` :320-351 (arrowhead Rayleigh-Ritz,
+smallest k unfiltered / largest k filtered), ``OOC7_A_RitzV_u_alpha_beta_istart_iend_VOID``
+:415-522 (Lanczos steps: alpha = v.u, CGS pass, beta = ||u|| taken before the second pass,
+CGS pass 2, v_next = u / beta).
+
+Device side: the basis V [ncv, n] float64 is row-major (one Lanczos vector per row); the
+projections are the multi-dot and rank-J update kernels (deterministic reductions); the
+restart combination V[:k] = W^T V is ``inch_rr_combine``. alpha = V[i].u is read off the
+first projection (h[i]) instead of a separate dot. Only the (ncv x ncv) projected
+eigenproblem goes to the host (numpy eigh), once per restart, as in the reference.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import ops
+
+
+def select_ritz(alpha, beta, beta_k, k, filtered):
+    t = np.diag(alpha) + np.diag(beta[:-1], 1) + np.diag(beta[:-1], -1)
+    if beta_k is not None:
+        t[k, :k] = beta_k
+        t[:k, k] = beta_k
+    S, W = np.linalg.eigh(t)
+    idx = np.argsort(S)
+    sel = idx[-k:][::-1] if filtered else idx[:k]
+    return S[sel], W[:, sel]
+
+
+class TRLMResult:
+    def __init__(self, eigval, eigvec, iters, res, n_spmv, seconds, history):
+        self.eigval = eigval
+        self.eigvec = eigvec
+        self.iters = iters
+        self.res = res
+        self.n_spmv = n_spmv
+        self.seconds = seconds
+        self.history = history
+
+
+class TRLM:
+    """Device TRLM on an ``Operator`` (see operator.py)."""
+
+    def __init__(self, op, k, factor=2, ncv=None):
+        n = op.n
+        if k >= n:
+            raise AssertionError("ABORTED. k must be smaller than n")
+        self.op = op
+        self.k = int(k)
+        self.n = n
+        self.ncv = int(ncv) if ncv else min(max(factor * self.k, self.k + 32), n - 1)
+        dev = op.A.device
+        self.V = torch.empty((self.ncv, n), dtype=torch.float64, device=dev)
+        self.Vk = torch.empty((self.k, n), dtype=torch.float64, device=dev)
+        self.u = torch.empty(n, dtype=torch.float64, device=dev)
+        self.h = torch.empty(self.ncv + 1, dtype=torch.float64, device=dev)
+        self.alpha = torch.zeros(self.ncv, dtype=torch.float64, device=dev)
+        self.beta = torch.zeros(self.ncv, dtype=torch.float64, device=dev)
+        self.scal = torch.empty(1, dtype=torch.float64, device=dev)
+        self.betak_d = torch.empty(self.k, dtype=torch.float64, device=dev)
+        self.ws = ops.Workspace(self.ncv + 1, dev)
+
+    # -- Lanczos steps i_start .. i_end-1 (the last step leaves its residual in u)
+    def lanczos(self, i_start, i_end):
+        V, u, h, ws = self.V, self.u, self.h, self.ws
+        for i in range(i_start, i_end):
+            self.op.apply(V[i], u)
+            ops.multidot(V, i + 1, u, h, ws)
+            self.alpha[i:i + 1].copy_(h[i:i + 1])
+            ops.gemv_update(V, i + 1, h, u, 1.0, nrm2_out=self.beta[i:i + 1], ws=ws)
+            ops.multidot(V, i + 1, u, h, ws)
+            ops.gemv_update(V, i + 1, h, u, 1.0)
+            if i >= i_end - 1:
+                break
+            ops.scale_div(u, self.beta[i:i + 1], V[i + 1])
+
+    def _restart_rr(self, alpha_h, beta_h, beta_k, filtered):
+        S, W = select_ritz(alpha_h, beta_h, beta_k, self.k, filtered)
+        Wd = torch.as_tensor(np.ascontiguousarray(W), device=self.V.device)
+        ops.rr_combine(self.V, self.ncv, Wd, self.Vk)
+        self.V[:self.k].copy_(self.Vk)
+        beta_k = beta_h[-1] * W[-1, :]
+        return S, beta_k, float(np.linalg.norm(beta_k))
+
+    def run(self, maxiter=10000, tol=0.0, v0=None, seed=0, filtered=False, check=None, verbose=False):
+        """Returns TRLMResult. ``check(self, S)`` (optional) is called after every restart and
+        may return True to stop (residual-verified acceptance); ``tol`` is the reference's
+        absolute criterion on ||beta_k||."""
+        t0 = time.time()
+        k, ncv = self.k, self.ncv
+        if tol == 0:
+            tol = np.finfo(np.float64).eps
+        if v0 is None:
+            g = torch.Generator(device=self.V.device)
+            g.manual_seed(int(seed))
+            self.u.copy_(torch.rand(self.n, generator=g, dtype=torch.float64, device=self.V.device))
+        else:
+            self.u.copy_(torch.as_tensor(v0, dtype=torch.float64, device=self.V.device).reshape(-1))
+        self.V[0].copy_(self.u / torch.linalg.norm(self.u))
+        self.lanczos(0, ncv)
+        alpha_h = self.alpha.cpu().numpy().copy()
+        beta_h = self.beta.cpu().numpy().copy()
+        S, beta_k, res = self._restart_rr(alpha_h, beta_h, None, filtered)
+        history = [res]
+        it = 0
+        stop = False
+        if check is not None:
+            stop = bool(check(self, S))
+        for it in range(maxiter):
+            if res <= tol or stop:
+                break
+            V, u, ws = self.V, self.u, self.ws
+            # restart vector: u orthogonalised against the k Ritz vectors
+            ops.multidot(V, k, u, self.h, ws)
+            ops.gemv_update(V, k, self.h, u, 1.0, nrm2_out=self.scal, ws=ws)
+            ops.scale_div(u, self.scal, V[k])
+            self.op.apply(V[k], u)
+            ops.multidot(V[k:k + 1], 1, u, self.alpha[k:k + 1], ws)
+            ops.axpy_dev(self.alpha[k:k + 1], -1.0, V[k], u)
+            self.betak_d.copy_(torch.as_tensor(beta_k, device=V.device))
+            ops.gemv_update(V, k, self.betak_d, u, 1.0, nrm2_out=self.beta[k:k + 1], ws=ws)
+            ops.scale_div(u, self.beta[k:k + 1], V[k + 1])
+            self.lanczos(k + 1, ncv)
+            alpha_h = self.alpha.cpu().numpy().copy()
+            beta_h = self.beta.cpu().numpy().copy()
+            alpha_h[:k] = S
+            beta_h[:k] = 0.0
+            S, beta_k, res = self._restart_rr(alpha_h, beta_h, beta_k, filtered)
+            history.append(res)
+            if verbose:
+                print("TRLM restart %d  ||beta_k|| = %.3e  spmv = %d" % (it, res, self.op.n_spmv))
+            if check is not None:
+                stop = bool(check(self, S))
+        S = np.asarray(S, dtype=np.float64)
+        torch.cuda.synchronize()
+        return TRLMResult(S, self.V[:k], it, res, self.op.n_spmv, time.time() - t0, history)
+
+
+def rayleigh_quotients(A, Vrows, precision="f64"):
+    """vi . (A vi) for the rows of Vrows (float64 [k, n]) with the unfiltered operator."""
+    X = Vrows.T.contiguous()
+    Y = A.matmat(X, precision=precision) if precision else A.matfree(X)
+    return (X * Y).sum(0)

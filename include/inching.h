@@ -1,0 +1,167 @@
+/*
+ * inching.h -- C ABI of libinching_hip.so, the MI355X (gfx950) implementation of
+ * Inching's eigenproblem construction + diagonalisation path.
+ *
+ * Conventions
+ *  - Every entry point returns inch_status (0 = OK, negative = error).
+ *  - Device pointers are plain pointers to memory the caller owns (allocated by
+ *    the caller, e.g. through torch); the library never frees caller memory.
+ *  - `stream` is a hipStream_t passed as void*; NULL means the default stream.
+ *    Launches are asynchronous on that stream unless the comment says "syncs".
+ *  - Atoms are in the caller's units (the reference uses nm with the cutoff in
+ *    Angstrom / 10); dof n = 3 * n_atoms, vectors interleave x,y,z per atom.
+ *  - Block matrices ("block rows") are full-storage 3x3 block CSR:
+ *      rowptr int64[n_atoms+1], col int32[nnzb] (ascending per row, diagonal
+ *      included), vals T[nnzb*9] row-major 3x3 per block.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository):
+ *  inch_neighbors_celllist   scipy cKDTree.query_ball_point calls of
+ *                            InchingLiteInteger/Fuel/Coordinate/T1.py:219,239,276-277,365,397
+ *  inch_rcm_host             X_KdCuthillMckeeOrder, Fuel/Coordinate/T1.py:171-323
+ *  inch_batch_head           GetPartitionTree/FlattenPartitionTree util.py:563-587 and
+ *                            Fuel/T1.py:145-153
+ *  inch_hessian_assemble     Xnumpy_SparseCupyMatrixUngappped.ReturnCupyHLowerTriangleInt32/
+ *                            Int64, Fuel/T1.py:228-660 / :666-1021
+ *  inch_bsr_spmv             OOC2_HalfMemS_v_KrylovAv_VOID / OOC2_FullMemS_v_KrylovAv_VOID
+ *                            closures, Burn/Krylov/T3.py:54-239 (plus the Hotelling term of
+ *                            Burn/Orthogonalization/T3.py:62-164 and the Chebyshev step of
+ *                            Burn/PolynomialFilters/T2.py:59-80 fused as epilogues)
+ *  inch_multidot / inch_gemv_update / inch_vec_*
+ *                            cuBLAS dot/nrm2/gemv calls of
+ *                            Burn/ThickRestartLanczosHotellingDeflation/T1.py:465-520 and
+ *                            Burn/Orthogonalization/T2.py:64-137
+ *  inch_rr_combine           V[:k] = W^T V, TRLM/T1.py:176,246; IRLM/T1.py:286-294; JDM/T1.py:553
+ *  inch_anm_matfree_apply    the residual check of the drivers,
+ *                            Command8A/commandBenchmark_InchingTRLMHD_Linux_Pdb_0064_A100.py:470-476
+ */
+#ifndef INCHING_H
+#define INCHING_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t inch_status;
+enum {
+  INCH_OK = 0,
+  INCH_ERR_INVALID = -1,      /* bad argument / shape */
+  INCH_ERR_HIP = -2,          /* a HIP runtime call failed */
+  INCH_ERR_CAPACITY = -3,     /* an output buffer or a fixed capacity is too small */
+  INCH_ERR_NOT_CONVERGED = -4,
+  INCH_ERR_NAN = -5
+};
+
+/* dtype codes */
+enum { INCH_F32 = 0, INCH_F64 = 1 };
+
+const char *inch_status_string(inch_status s);
+const char *inch_last_error(void);   /* last error message of this thread */
+int32_t inch_abi_version(void);
+
+/* ---------------------------------------------------------------- host ---- */
+
+/* Seeded random-sequential-addition globule: n atoms uniform in a sphere of the
+ * given radius centred at the origin with pairwise distance >= dmin (cell grid,
+ * xoshiro256** stream seeded by splitmix64(seed)). Writes xyz[n*3] (float64).
+ * Returns INCH_ERR_CAPACITY if the sphere saturates before n atoms are placed
+ * (after max_attempts darts). Host only. */
+inch_status inch_synth_globule(int64_t n, double radius, double dmin, uint64_t seed,
+                               int64_t max_attempts, double *xyz);
+
+/* Reverse Cuthill-McKee on a neighbour CSR (self included, ascending columns).
+ * seed_order: optional int64[n] seed order (NULL = stable argsort of degree).
+ * order[n] (int32) = new->old; undo[n] (int64) = argsort(order). Host only. */
+inch_status inch_rcm_host(int64_t n, const int64_t *indptr, const int32_t *indices,
+                          const int64_t *seed_order, int32_t reverse, int32_t *order,
+                          int64_t *undo);
+
+/* Row-batch heads of recursive halving with leaves <= maxleaf, sorted descending
+ * and cumulated: head[0..nbatch]. cap = capacity of head. Host only. */
+inch_status inch_batch_head(int64_t n, int64_t maxleaf, int64_t *head, int64_t cap,
+                            int64_t *nbatch);
+
+/* -------------------------------------------------------------- device ---- */
+
+/* Cell-list neighbour search on the device. xyz: device float64[n*3].
+ * Phase 1 (indices == NULL): writes indptr[n+1] (int64, device) and returns the
+ *   total count in *nnz_out (syncs).
+ * Phase 2 (indices != NULL, capacity >= nnz): fills indices (int32, ascending
+ *   per row, self included). Neighbour rule: (dx*dx+dy*dy)+dz*dz <= rc*rc in
+ *   float64 without fused multiply-add. Rows with more than 2048 neighbours
+ *   return INCH_ERR_CAPACITY. */
+inch_status inch_neighbors_celllist(int64_t n, const double *xyz, double rc, int64_t *indptr,
+                                    int32_t *indices, int64_t capacity, int64_t *nnz_out,
+                                    void *stream);
+
+/* ANM Hessian A = H + plus_i*I into full-storage block rows on the pattern
+ * (rowptr, col) (the neighbour CSR). vals64 (float64[nnzb*9]) and/or vals32
+ * (float32[nnzb*9], round-to-nearest of the float64 value) may be NULL.
+ * diag (float64[3n], optional) receives the scalar diagonal A_diag. */
+inch_status inch_hessian_assemble(int64_t n, const double *xyz, double rc, double plus_i,
+                                  const int64_t *rowptr, const int32_t *col, double *vals64,
+                                  float *vals32, double *diag, void *stream);
+
+/* Block SpMV with fused epilogue (all vectors length 3n, same dtype vdt):
+ *   w   = A x                                   (A in dtype mdt, accumulate in mdt)
+ *   out = s1 * (w - c0 * x) - c2 * z            (z may be NULL => no z term)
+ *   if qcoef: out += sum_m qcoef[m] * Q[m, :]   (Q float64[m_q, 3n], qcoef float64 device)
+ *   y   = out;  if acc: acc += mu * out
+ * Plain y = A x is s1 = 1, c0 = 0, z = acc = qcoef = NULL. */
+inch_status inch_bsr_spmv(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals,
+                          int32_t mdt, int32_t vdt, const void *x, void *y, double s1, double c0,
+                          const void *z, double c2, void *acc, double mu, const double *Q,
+                          const double *qcoef, int32_t m_q, int32_t lanes_per_row, void *stream);
+
+/* Reference half-storage views of the same block matrix: part 0 = A, 1 = L (scalar
+ * lower triangle incl. diagonal, what the reference's CSR `A @ v` computes), 2 = L^T
+ * (`A.T @ v`). Compatibility path, not tuned. */
+inch_status inch_bsr_spmv_part(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals,
+                               int32_t mdt, int32_t vdt, const void *x, void *y, int32_t part, void *stream);
+
+/* Several right-hand sides: Y[:, r] = A X[:, r] for r < nrhs, X/Y row-major
+ * [3n, nrhs] (the nrhs values of one dof are contiguous). */
+inch_status inch_bsr_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals,
+                          int32_t mdt, int32_t vdt, const void *X, void *Y, int32_t nrhs,
+                          void *stream);
+
+/* Matrix-free FP64 ANM operator recomputed from float64 coordinates and the
+ * neighbour CSR: y = (H + plus_i I) x. Used for the FP64 residual polish. */
+inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *rowptr,
+                                   const int32_t *col, double plus_i, const double *x, double *y,
+                                   int32_t nrhs, void *stream);
+
+/* h[j] = sum_i V[j*ld + i] * u[i] for j < J (float64, deterministic two-phase
+ * reduction). work: float64 scratch of inch_multidot_work_size(J) elements. */
+int64_t inch_multidot_work_size(int32_t J);
+inch_status inch_multidot(int64_t len, int32_t J, const double *V, int64_t ld, const double *u,
+                          double *h, double *work, void *stream);
+
+/* u[i] -= scale * sum_j V[j*ld + i] * h[j]  (h on device).
+ * If nrm2_out: *nrm2_out = ||u_new||_2 (two-phase, work as in inch_multidot). */
+inch_status inch_gemv_update(int64_t len, int32_t J, const double *V, int64_t ld, const double *h,
+                             double scale, double *u, double *nrm2_out, double *work, void *stream);
+
+/* dst[i] = src[i] / (*div)   (div: device scalar), optionally also into dst2. */
+inch_status inch_vec_scale_div(int64_t len, const double *src, const double *div, double *dst,
+                               double *dst2, void *stream);
+/* y[i] += a[0] * x[i] * sign  (a: device scalar). */
+inch_status inch_vec_axpy_dev(int64_t len, const double *a, double sign, const double *x, double *y,
+                              void *stream);
+/* y = a*x + b*y elementwise (any float32/float64 combination; b == 0 ignores y). */
+inch_status inch_vec_axpby(int64_t len, double a, const void *x, int32_t xdt, double b, void *y, int32_t ydt,
+                           void *stream);
+/* Type conversion between float64 and float32 vectors. */
+inch_status inch_vec_convert(int64_t len, const void *src, int32_t sdt, void *dst, int32_t ddt,
+                             void *stream);
+
+/* out[q*ld_out + i] = sum_r W[r*k + q] * V[r*ld + i]  for q < k, r < m
+ * (W float64 [m, k] row-major on device; out must not alias V). */
+inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, int64_t ld,
+                            const double *W, double *out, int64_t ld_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INCHING_H */

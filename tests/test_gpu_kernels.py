@@ -1,0 +1,219 @@
+"""GPU parity of the construction and operator kernels against the CPU oracle.
+
+Integer/index outputs (neighbour CSR, RCM order) must be bit-exact; float64 values agree
+to rounding (the diagonal sums run in a different order); float32 values are the
+round-to-nearest of the float64 values (<= 1 ulp); float32 SpMV has relative error
+<= 1e-6 in the 2-norm.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from oracle import geometry as OG
+from oracle import hessian as OH
+from oracle import rcm as OR
+from tests.helpers import small_globule
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=None):
+    t = torch.as_tensor(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+@pytest.fixture(scope="module")
+def glob2k(cuda):
+    X = small_globule(2000, seed=21)
+    return X
+
+
+@pytest.fixture(scope="module")
+def glob40k(cuda):
+    from inching_amd import synth
+    return synth.globule(40000, 3)
+
+
+def _print(*a):
+    print(*a, flush=True)
+
+
+def test_neighbours_bit_exact_small(glob2k):
+    from inching_amd import ops
+    _print("case: neighbours n=2000 rc=8")
+    ip_ref, ix_ref = OG.neighbour_csr(glob2k, 8.0)
+    ip, ix = ops.neighbours(_dev(glob2k), 8.0)
+    assert np.array_equal(ip.cpu().numpy(), ip_ref)
+    assert np.array_equal(ix.cpu().numpy(), ix_ref)
+
+
+def test_neighbours_bit_exact_40k_nm_units(glob40k):
+    from inching_amd import ops
+    _print("case: neighbours n=40000 in nm, rc=0.8")
+    Xnm = glob40k / 10.0
+    ip_ref, ix_ref = OG.neighbour_csr(Xnm, 0.8)
+    ip, ix = ops.neighbours(_dev(Xnm), 0.8)
+    assert np.array_equal(ip.cpu().numpy(), ip_ref)
+    assert np.array_equal(ix.cpu().numpy(), ix_ref)
+
+
+def test_rcm_pipeline_matches_oracle(glob2k):
+    from inching_amd.hessian import rcm_order
+    _print("case: rcm pipeline n=2000")
+    ip, ix = OG.neighbour_csr(glob2k, 8.0)
+    o_ref, u_ref = OR.rcm_order(ip, ix)
+    o, u = rcm_order(glob2k, 8.0)
+    assert np.array_equal(o, o_ref) and np.array_equal(u, u_ref)
+
+
+def test_assembly_matches_oracle(glob2k):
+    from inching_amd.hessian import build_hessian
+    _print("case: assembly n=2000")
+    A = build_hessian(glob2k, 8.0, plus_i=1.0, reorder=True, precision=("f32", "f64"))
+    Xp = glob2k[A.order]
+    rp_ref, col_ref, v_ref = OH.block_hessian(Xp, 8.0, 1.0)
+    assert np.array_equal(A.rowptr.cpu().numpy(), rp_ref)
+    assert np.array_equal(A.col.cpu().numpy(), col_ref)
+    v64 = A.vals64.cpu().numpy().reshape(-1, 3, 3)
+    assert np.max(np.abs(v64 - v_ref)) <= 1e-13 * max(1.0, np.max(np.abs(v_ref)))
+    v32 = A.vals32.cpu().numpy().reshape(-1, 3, 3)
+    ulp = np.spacing(np.abs(v_ref).astype(np.float32))
+    assert np.all(np.abs(v32.astype(np.float64) - v_ref) <= ulp)
+    d_ref = np.concatenate([np.diag(v) for v in v_ref[col_ref == np.repeat(np.arange(2000), np.diff(rp_ref))]])
+    assert np.allclose(A.A_diag.cpu().numpy(), d_ref, rtol=1e-13, atol=1e-13)
+
+
+def test_assembly_invariants_40k(glob40k):
+    from inching_amd.hessian import build_hessian, rigid_body_basis
+    _print("case: assembly invariants n=40000")
+    A = build_hessian(glob40k, 8.0, plus_i=1.0, reorder=True, precision=("f64",))
+    Q = rigid_body_basis(A.xyz)
+    AQ = A.matmat(Q.T.contiguous(), precision="f64")
+    # rigid-body modes are eigenvectors of A = H + I with eigenvalue 1
+    assert torch.max(torch.abs(AQ - Q.T)).item() < 1e-10
+    # trace identity: tr(H) = 2 * #pairs
+    n_pairs = (A.nnzb - A.n_atoms) // 2
+    assert abs((A.A_diag.sum().item() - 3 * A.n_atoms) - 2 * n_pairs) < 1e-6 * n_pairs
+
+
+@pytest.fixture(scope="module")
+def mat(glob2k):
+    from inching_amd.hessian import build_hessian
+    A = build_hessian(glob2k, 8.0, plus_i=1.0, reorder=True, precision=("f32", "f64"))
+    rp = A.rowptr.cpu().numpy()
+    col = A.col.cpu().numpy()
+    Bs = sp.bsr_matrix((A.vals64.cpu().numpy().reshape(-1, 3, 3), col.astype(np.int64), rp), shape=A.shape).tocsr()
+    return A, Bs
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_spmv_f64_f32(mat, lanes):
+    from inching_amd import ops
+    A, Bs = mat
+    _print("case: spmv lanes=%d" % lanes)
+    x = np.random.default_rng(lanes).standard_normal(A.shape[0])
+    ref = Bs @ x
+    y = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, _dev(x), y, lanes=lanes)
+    assert np.linalg.norm(y.cpu().numpy() - ref) <= 1e-13 * np.linalg.norm(ref)
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, _dev(x), y, lanes=lanes)   # f32 matrix, f64 vectors
+    assert np.linalg.norm(y.cpu().numpy() - ref) <= 1e-6 * np.linalg.norm(ref)
+    y32 = torch.empty(A.shape[0], dtype=torch.float32, device="cuda")
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, _dev(x, torch.float32), y32, lanes=lanes)
+    assert np.linalg.norm(y32.cpu().numpy().astype(np.float64) - ref) <= 1e-6 * np.linalg.norm(ref)
+
+
+def test_spmv_deterministic(mat):
+    from inching_amd import ops
+    A, _ = mat
+    x = _dev(np.random.default_rng(0).standard_normal(A.shape[0]), torch.float32)
+    y1 = torch.empty_like(x)
+    y2 = torch.empty_like(x)
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x, y1)
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x, y2)
+    assert torch.equal(y1, y2)
+
+
+def test_spmv_epilogues(mat):
+    from inching_amd import ops
+    from inching_amd.hessian import rigid_body_basis
+    A, Bs = mat
+    _print("case: spmv epilogues")
+    rng = np.random.default_rng(3)
+    n = A.shape[0]
+    x, z, acc = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(n)
+    s1, c0, c2, mu = 0.07, 12.5, 1.0, -0.3
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    accd = _dev(acc)
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, _dev(x), y, s1=s1, c0=c0, z=_dev(z), c2=c2, acc=accd, mu=mu)
+    out = s1 * (Bs @ x - c0 * x) - c2 * z
+    assert np.allclose(y.cpu().numpy(), out, rtol=1e-12, atol=1e-12)
+    assert np.allclose(accd.cpu().numpy(), acc + mu * out, rtol=1e-12, atol=1e-12)
+    Q = rigid_body_basis(A.xyz)
+    qc = _dev(rng.standard_normal(6))
+    ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, _dev(x), y, Q=Q, qcoef=qc)
+    ref = Bs @ x + Q.cpu().numpy().T @ qc.cpu().numpy()
+    assert np.allclose(y.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_reference_half_storage_views(mat):
+    A, Bs = mat
+    x = np.random.default_rng(4).standard_normal(A.shape[0])
+    L = sp.tril(Bs, format="csr")
+    xd = _dev(x)
+    assert np.allclose((A @ xd).cpu().numpy(), L @ x, rtol=1e-13, atol=1e-13)
+    assert np.allclose((A.T @ xd).cpu().numpy(), L.T @ x, rtol=1e-13, atol=1e-13)
+    full = (A @ xd) + (A.T @ xd) - A.diagonal(0) * xd
+    assert np.allclose(full.cpu().numpy(), Bs @ x, rtol=1e-12, atol=1e-12)
+
+
+def test_spmm_and_matfree(mat):
+    A, Bs = mat
+    X = np.random.default_rng(5).standard_normal((A.shape[0], 11))
+    Y = A.matmat(_dev(X), precision="f64").cpu().numpy()
+    assert np.allclose(Y, Bs @ X, rtol=1e-12, atol=1e-12)
+    Y32 = A.matmat(_dev(X, torch.float32), precision="f32").cpu().numpy()
+    assert np.linalg.norm(Y32 - Bs @ X) <= 1e-6 * np.linalg.norm(Bs @ X)
+    Ym = A.matfree(_dev(X)).cpu().numpy()
+    assert np.linalg.norm(Ym - Bs @ X) <= 1e-13 * np.linalg.norm(Bs @ X)
+
+
+def test_dense_kernels(cuda):
+    from inching_amd import ops
+    rng = np.random.default_rng(6)
+    n, J = 100003, 37
+    V = rng.standard_normal((J + 3, n))
+    u = rng.standard_normal(n)
+    Vd, ud = _dev(V), _dev(u)
+    ws = ops.Workspace(64, Vd.device)
+    h = torch.empty(64, dtype=torch.float64, device="cuda")
+    ops.multidot(Vd, J, ud, h, ws)
+    assert np.allclose(h[:J].cpu().numpy(), V[:J] @ u, rtol=1e-12, atol=1e-9)
+    nrm = torch.empty(1, dtype=torch.float64, device="cuda")
+    ops.gemv_update(Vd, J, h, ud, 0.5, nrm2_out=nrm, ws=ws)
+    ref = u - 0.5 * V[:J].T @ (V[:J] @ u)
+    assert np.allclose(ud.cpu().numpy(), ref, rtol=1e-11, atol=1e-9)
+    assert np.isclose(nrm.item(), np.linalg.norm(ref), rtol=1e-12)
+    W = rng.standard_normal((J, 21))
+    out = torch.empty((21, n), dtype=torch.float64, device="cuda")
+    ops.rr_combine(Vd, J, _dev(W), out)
+    assert np.allclose(out.cpu().numpy(), W.T @ V[:J], rtol=1e-11, atol=1e-9)
+    W2 = rng.standard_normal((J, 5))
+    out2 = torch.empty((5, n), dtype=torch.float64, device="cuda")
+    ops.rr_combine(Vd, J, _dev(W2), out2)
+    assert np.allclose(out2.cpu().numpy(), W2.T @ V[:J], rtol=1e-11, atol=1e-9)
+    dst, dst2 = torch.empty(n, dtype=torch.float64, device="cuda"), torch.empty(n, dtype=torch.float64, device="cuda")
+    ops.scale_div(ud, nrm, dst, dst2)
+    assert torch.allclose(dst, ud / nrm) and torch.equal(dst, dst2)
+    a = _dev(np.array([0.25]))
+    y = _dev(u)
+    ops.axpy_dev(a, -1.0, Vd[0], y)
+    assert np.allclose(y.cpu().numpy(), u - 0.25 * V[0])
+    y32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    ops.axpby(2.0, Vd[1], 0.0, y32)
+    assert np.allclose(y32.cpu().numpy(), (2.0 * V[1]).astype(np.float32))
+    ops.convert(y32, dst)
+    assert np.array_equal(dst.cpu().numpy(), y32.cpu().numpy().astype(np.float64))

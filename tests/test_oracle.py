@@ -1,0 +1,292 @@
+"""Pins the CPU oracle: golden vectors, library routines, closed forms and invariants."""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+from oracle import chebyshev as OC
+from oracle import geometry as OG
+from oracle import hessian as OH
+from oracle import lanczos as OL
+from oracle import partition as OP
+from oracle import rcm as OR
+from tests.helpers import small_globule, subspace_sines
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------ geometry --
+def test_neighbours_kdtree_equals_bruteforce():
+    rng = np.random.default_rng(1)
+    X = rng.uniform(0, 25, size=(600, 3))
+    a = OG.neighbour_csr(X, 6.5)
+    b = OG.neighbour_csr_bruteforce(X, 6.5)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    # symmetric, self included
+    ip, ix = a
+    rows = np.repeat(np.arange(600), np.diff(ip))
+    S = sp.csr_matrix((np.ones(ix.size), (rows, ix)), shape=(600, 600))
+    assert (S != S.T).nnz == 0
+    assert np.all(S.diagonal() == 1)
+
+
+# ----------------------------------------------------------------------- RCM --
+def test_rcm_path_graph_known_answer():
+    # path 0-1-2-3-4 (self included): seeds by stable degree -> 0; BFS 0,1,2,3,4; reversed
+    ip = np.array([0, 2, 5, 8, 11, 13])
+    ix = np.array([0, 1, 0, 1, 2, 1, 2, 3, 2, 3, 4, 3, 4])
+    order, undo = OR.rcm_order(ip, ix)
+    assert order.tolist() == [4, 3, 2, 1, 0]
+    assert np.array_equal(undo, np.argsort(order))
+
+
+def test_rcm_children_sorted_by_degree_then_descending_index():
+    # star centre 0 with leaves 1..4; leaf 2 also touches 5 (higher degree)
+    edges = [(0, 1), (0, 2), (0, 3), (0, 4), (2, 5)]
+    n = 6
+    adj = [set([i]) for i in range(n)]
+    for a, b in edges:
+        adj[a].add(b)
+        adj[b].add(a)
+    ip = np.zeros(n + 1, dtype=np.int64)
+    ip[1:] = np.cumsum([len(s) for s in adj])
+    ix = np.concatenate([sorted(s) for s in adj])
+    order, _ = OR.rcm_order(ip, ix, reverse=False)
+    # seed = first min-degree node (stable): degrees [5,2,3,2,2,2] -> node 1
+    # BFS from 1: neighbours desc [1,0] -> 0; from 0: desc [4,3,2,1,0] new 4,3,2 sorted by degree (2,2,3) -> 4,3,2
+    # from 4: none; 3: none; 2: desc [5,2,0] -> 5
+    assert order.tolist() == [1, 0, 4, 3, 2, 5]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_rcm_matches_scipy_reverse_cuthill_mckee(seed):
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(0, 30, size=(900, 3))
+    ip, ix = OG.neighbour_csr(X, 6.0)
+    n = len(ip) - 1
+    ixd = np.concatenate([ix[ip[i]:ip[i + 1]][::-1] for i in range(n)])
+    G = sp.csr_matrix((np.ones(len(ixd)), ixd, ip), shape=(n, n))
+    ref = reverse_cuthill_mckee(G, symmetric_mode=True)
+    deg = np.diff(ip)
+    order, _ = OR.rcm_order(ip, ix, seed_order=np.argsort(deg))
+    assert np.array_equal(order, ref)
+
+
+def test_rcm_disconnected_components_and_bandwidth():
+    rng = np.random.default_rng(3)
+    A = rng.uniform(0, 15, size=(200, 3))
+    B = rng.uniform(100, 115, size=(150, 3))
+    X = np.concatenate([A, B])
+    ip, ix = OG.neighbour_csr(X, 5.0)
+    order, undo = OR.rcm_order(ip, ix)
+    assert sorted(order.tolist()) == list(range(350))
+    assert OR.bandwidth(ip, ix, order) < OR.bandwidth(ip, ix, rng.permutation(350))
+
+
+# ----------------------------------------------------------------- partition --
+def test_batch_head_known_counts():
+    # 98,543 atoms with leaves <= 100 gave 1024 batches in the reference's notebook output
+    # (Notebook/Application/01_Inching_EigenproblemJDMHD.ipynb cell 4)
+    assert len(OP.batch_head(98543, 100)) - 1 == 1024
+    for n in (1, 99, 100, 101, 5000, 123457):
+        h = OP.batch_head(n, 100)
+        sizes = np.diff(h)
+        assert h[0] == 0 and h[-1] == n
+        assert np.all(sizes <= 100) and np.all(sizes[:-1] >= sizes[1:])
+        if n > 100:
+            assert np.all(sizes > 25)
+
+
+def test_islands_cover_all_columns():
+    X = small_globule(700, seed=4)
+    ip, ix = OG.neighbour_csr(X, 8.0)
+    order, _ = OR.rcm_order(ip, ix)
+    ip, ix = OG.neighbour_csr(X[order], 8.0)
+    isl, half = OP.ungapped_islands(ip, ix, maxleafsize=50, gap_size=20)
+    head = OP.batch_head(700, 50)
+    assert half == ((ix.size - 700) / 2 + 700) * 9
+    for b in range(len(head) - 1):
+        cols = np.unique(ix[ip[head[b]]:ip[head[b + 1]]])
+        covered = np.zeros(cols.size, bool)
+        prev_end = -1
+        for (c0, c1) in isl[b]:
+            assert c0 < c1 and c0 >= prev_end
+            prev_end = c1
+            covered |= (cols >= c0) & (cols < c1)
+        assert covered.all()
+        assert isl[b][0][0] == cols[0] and isl[b][-1][1] == cols[-1] + 1
+
+
+# ------------------------------------------------------------------- Hessian --
+def test_two_atom_hessian_closed_form():
+    X = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 2.0]])
+    H = OH.hessian_dense(X, 5.0, plus_i=0.0)
+    e = X[1] - X[0]
+    E = np.outer(e, e) / (e @ e)
+    ref = np.block([[E, -E], [-E, E]])
+    assert np.allclose(H, ref, rtol=0, atol=1e-15)
+
+
+def test_hessian_invariants():
+    X = small_globule(400, seed=5)
+    A = OH.hessian_dense(X, 8.0, plus_i=1.0)
+    H = A - np.eye(A.shape[0])
+    assert np.allclose(H, H.T, atol=1e-14)
+    Q = OH.rigid_body_basis(X)
+    assert np.allclose(Q @ Q.T, np.eye(6), atol=1e-12)
+    assert np.max(np.abs(H @ Q.T)) < 1e-11
+    ip, ix = OG.neighbour_csr(X, 8.0)
+    n_pairs = (ix.size - 400) // 2
+    assert np.isclose(np.trace(H), 2 * n_pairs, rtol=1e-12)
+    w = np.linalg.eigvalsh(H)
+    assert w[0] > -1e-10 and np.sum(np.abs(w) < 1e-9) == 6
+
+
+def test_lower_csr_and_halfmem_matvec():
+    X = small_globule(300, seed=6)
+    L, d = OH.lower_csr_and_diag(X, 8.0, plus_i=1.0)
+    A = OH.hessian_dense(X, 8.0, plus_i=1.0)
+    assert np.allclose(L.toarray(), np.tril(A), atol=0)
+    assert np.allclose(d, np.diag(A))
+    v = np.random.default_rng(0).standard_normal(900)
+    assert np.allclose(OH.halfmem_matvec(L, d, v), A @ v, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------------------- Chebyshev --
+def _golden():
+    g = np.load(os.path.join(GOLD, "cheb_design.npz"))
+    return g, json.loads(bytes(g["meta"]).decode())
+
+
+def test_chebyshev_design_matches_reference_golden():
+    g, meta = _golden()
+    for ci, m in enumerate(meta):
+        maxd, mind, ker, ext, wanted, bound, ratio = m["case"]
+        pol = OC.optimize_on_degree(maxd, mind, ker, ext, tuple(wanted), tuple(bound), ratio)
+        assert pol.AdjustedDegree == m["degree"]
+        assert np.array_equal(np.asarray(pol.AdjustedCoefficients), g["mu_%d" % ci])
+        assert np.array_equal(pol.AdjustedDampingKernels, g["jac_%d" % ci])
+        assert float(pol.gam) == m["gam"] and float(pol.cc) == m["cc"] and float(pol.dd) == m["dd"]
+
+
+def test_damping_kernel_closed_form():
+    g, _ = _golden()
+    for m in (5, 17, 100):
+        k = np.arange(1, m + 1)
+        th = np.pi / (m + 2)
+        jack = ((1 - k / (m + 2)) * np.sin(th) * np.cos(k * th) + np.cos(th) * np.sin(k * th) / (m + 2)) / np.sin(th)
+        # the reference codes a1*sin((k+1)th)/sin th + (1-(k+1)a1) cos(k th); compare at tolerance
+        coded = np.sin((k + 1) * th) / ((m + 2) * np.sin(th)) + (1 - (k + 1) / (m + 2)) * np.cos(k * th)
+        assert np.allclose(g["damp_Jackson_%d" % m][1:m + 1], coded, rtol=1e-13, atol=1e-15)
+        assert g["damp_Jackson_%d" % m][0] == 0.5
+        sig = np.sin(k * np.pi / (m + 1)) / (k * np.pi / (m + 1))
+        assert np.allclose(g["damp_LanczosSigma_%d" % m][1:m + 1], sig, rtol=1e-14)
+        assert np.all(g["damp_None_%d" % m][1:] == 1.0)
+        assert jack.shape == coded.shape
+        assert np.array_equal(OC.damping_kernel(m, 200, "Jackson"), g["damp_Jackson_%d" % m])
+
+
+def test_cheb_iv_equals_chebval():
+    rng = np.random.default_rng(0)
+    mu = rng.standard_normal(30)
+    x = np.linspace(-1, 1, 41)
+    assert np.allclose(OC.cheb_iv(29, mu, x), np.polynomial.chebyshev.chebval(x, mu), rtol=1e-12, atol=1e-12)
+
+
+def test_interior_filter_response():
+    pol = OC.optimize_on_degree(1000, 10, "Jackson", 1e-9, (5.0, 7.0), (1.0, 40.0), 0.1)
+    m = pol.AdjustedDegree
+    mu = np.asarray(pol.AdjustedCoefficients)[:m + 1]
+    xs = (np.array([5.0, 7.0]) - pol.cc) / pol.dd
+    val_edges = np.polynomial.chebyshev.chebval(xs, mu)
+    val_centre = np.polynomial.chebyshev.chebval(pol.gam, mu)
+    assert np.isclose(val_centre, 1.0, rtol=1e-10)
+    assert np.all(val_edges <= 0.1 + 1e-12)
+
+
+def _small_problem(n_atoms=400, seed=7):
+    X = small_globule(n_atoms, seed=seed)
+    A = OH.hessian_dense(X, 8.0, plus_i=1.0)
+    return X, A
+
+
+def test_cheb_av_equals_spectral_filter():
+    X, A = _small_problem(200, 8)
+    w, U = np.linalg.eigh(A)
+    pol = OC.optimize_on_degree(500, 5, "Jackson", 1e-10, (1.0, 3.0), (1.0, w[-1] + 1e-12), 0.7)
+    v = np.random.default_rng(1).standard_normal(A.shape[0])
+    y = OC.cheb_av(lambda x: A @ x, v, pol)
+    m = pol.AdjustedDegree
+    p = np.polynomial.chebyshev.chebval((w - pol.cc) / pol.dd, np.asarray(pol.AdjustedCoefficients)[:m + 1])
+    assert np.allclose(y, U @ (p * (U.T @ v)), rtol=1e-9, atol=1e-9)
+
+
+def test_kpm_count_is_close_to_exact():
+    X, A = _small_problem(300, 9)
+    w = np.linalg.eigvalsh(A)
+    pol = OC.optimize_on_degree(2000, 5, "Jackson", 1e-10, (1.0, 2.0), (1.0, w[-1] + 1e-12), 0.7)
+    est = OC.kpm_estimate(lambda x: A @ x, A.shape[0], pol, 40, np.random.default_rng(0))
+    exact = int(np.sum((w >= 1.0 - 1e-9) & (w <= 2.0)))
+    assert abs(est - exact) <= max(10, 0.35 * exact)
+
+
+# -------------------------------------------------------------- eigensolvers --
+def test_spectrum_bound_below_and_close():
+    X, A = _small_problem(300, 10)
+    w = np.linalg.eigvalsh(A)
+    lo, hi = OL.spectrum_bound(lambda x: A @ x, A.shape[0], np.random.default_rng(0), 200)
+    assert hi <= w[-1] * (1 + 1e-10) and hi >= w[-1] * (1 - 1e-6)
+    assert lo >= w[0] - 1e-8
+
+
+def _check_pairs(A, lam, V, w_ref, U_ref, k, tol_val, tol_res):
+    assert np.allclose(lam, w_ref[:k], rtol=0, atol=tol_val)
+    R = A @ V - V * lam[None, :]
+    assert np.max(np.linalg.norm(R, axis=0)) < tol_res
+    # the subspace of all returned modes matches (clusters resolved as a whole)
+    s = subspace_sines(U_ref[:, :k], V / np.linalg.norm(V, axis=0))
+    assert s.max() < 1e-4
+
+
+def test_trlm_hotelling_matches_dense_eigh():
+    X, A = _small_problem(400, 11)
+    Q = OH.rigid_body_basis(X)
+    w, U = np.linalg.eigh(A)
+    k = 12
+    lam, V, it = OL.trlm(lambda x: A @ x, A.shape[0], k, np.random.default_rng(0), tol=1e-10, Q=Q, shift=10.0)
+    _check_pairs(A, lam, V, w[6:], U[:, 6:], k, 1e-9, 1e-8)
+
+
+def test_trlm_filtered_matches_dense_eigh():
+    X, A = _small_problem(400, 12)
+    Q = OH.rigid_body_basis(X)
+    w, U = np.linalg.eigh(A)
+    k = 10
+    lam_max = w[-1] + 1e-12
+    pol = OC.optimize_on_degree(2000, 5, "Jackson", 1e-10, (1.0, w[6 + k + 6]), (1.0, lam_max), 0.7)
+    lam, V, it = OL.trlm(lambda x: A @ x, A.shape[0], k, np.random.default_rng(0), tol=1e-9, Q=Q, shift=-40.0,
+                         pol=pol)
+    _check_pairs(A, lam, V, w[6:], U[:, 6:], k, 1e-9, 1e-7)
+
+
+def test_jdm_matches_dense_eigh():
+    X, A = _small_problem(300, 13)
+    Q = OH.rigid_body_basis(X)
+    w, U = np.linalg.eigh(A)
+    k = 8
+    lam, V, it = OL.jdm(lambda x: A @ x, A.shape[0], k, np.random.default_rng(0), tol=1e-9, maxiter=2000, Q=Q,
+                        shift=40.0, gap=0.0)
+    _check_pairs(A, lam, V, w[6:], U[:, 6:], k, 1e-9, 1e-8)
+
+
+def test_irlm_matches_dense_eigh():
+    X, A = _small_problem(400, 14)
+    Q = OH.rigid_body_basis(X)
+    w, U = np.linalg.eigh(A)
+    k = 32
+    lam, V, it = OL.irlm(lambda x: A @ x, A.shape[0], k, np.random.default_rng(0), tol=1e-10, Q=Q, shift=10.0,
+                         maxiter=3000)
+    assert np.allclose(lam, w[6:6 + k], atol=1e-7)
