@@ -1,0 +1,194 @@
+#!/usr/bin/env python
+"""Time-to-k-modes benchmark on one MI355X.
+
+Workload (default): 300,000-atom seeded globule (0.058 atoms/A^3, 2.2 A minimum
+separation, seed 1), ANM cutoff 8 A, gamma 1, PlusI 1; lowest 64 non-trivial modes at
+relative residual ||A v - lam v|| / (lam_max ||v||) <= 1e-8 (float64 check), rigid-body
+block deflated by a Hotelling shift.
+
+One step = one full solve from coordinates resident in HBM: neighbour search, RCM
+ordering, Hessian assembly, rigid-body basis, spectrum bound, KPM-sized Chebyshev filter,
+Chebyshev-filtered TRLM, float64 polish and residual check. ``value`` is the mean wall
+time per step (seconds, lower is better) of the headline mode; the plain float64 solver
+and the other operator modes are timed in the same run and reported beside it.
+
+The ``roofline`` object describes the kernel with the largest total device time in the
+timed steps, from HIP events around each of its launches on the launch stream.
+``cpu_baseline`` times the CPU oracle (numpy/scipy restatement of the reference path) on
+a bounded sample of the same workload and scales it to the GPU run's operation counts.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--atoms", type=int, default=300000)
+    ap.add_argument("--k", type=int, default=64)
+    ap.add_argument("--mode", default="mixed", help="headline operator mode: mixed | mf64 | f64")
+    ap.add_argument("--compare", default="f64,mf64", help="other modes timed once each")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
+    return ap.parse_args()
+
+
+def kernel_bytes(kind, nnzb, n_atoms, epi):
+    """Algorithmic bytes of one operator launch (values + vectors; no index bytes)."""
+    nd = 3 * n_atoms
+    if kind == "f32":
+        s, sv = 4, 4 if epi == "cheb" else 8
+        return 9 * s * nnzb + (5 if epi == "cheb" else 2) * nd * sv
+    if kind == "f64":
+        return 9 * 8 * nnzb + (5 if epi == "cheb" else 2) * nd * 8
+    # matrix-free: neighbour indices + coordinates + vectors
+    return 4 * nnzb + 8 * (n_atoms + 1) + 24 * n_atoms + (5 if epi == "cheb" else 2) * nd * 8
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not supported (single-GPU path, "
+              "see DESIGN.md for how the solve would shard)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from inching_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        from inching_amd.build import build
+        build(verbose=False)
+    from inching_amd import ops, synth
+    from inching_amd.hessian import build_hessian, rigid_body_basis
+    from inching_amd.pipeline import lowest_modes
+
+    dev = torch.device("cuda", 0)
+    X = synth.globule(args.atoms, seed=1)
+    xyz = torch.as_tensor(X, device=dev)
+    torch.cuda.synchronize()
+
+    def step(mode, timer=None):
+        prec = {"mixed": ("f32",), "f64": ("f64",), "mf64": ()}[mode]
+        A = build_hessian(xyz, 8.0, plus_i=1.0, reorder=True, precision=prec)
+        Q = rigid_body_basis(A.xyz)
+        r = lowest_modes(A, args.k, Q=Q, precision=mode, tol_rel=args.tol, timer=timer, seed=0)
+        return A, r
+
+    # warm-up (also JIT-free: the library is prebuilt; this pages in code and the allocator)
+    for w in range(args.warmup):
+        step(args.mode)
+        print("bench: warmup %d done" % w, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    timer = []
+    results = []
+    t0 = time.time()
+    for s_ in range(args.steps):
+        results.append(step(args.mode, timer=timer))
+        print("bench: step %d done" % s_, file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    sec = (t1 - t0) / args.steps
+    A, r = results[-1]
+    nnzb, n_atoms = A.nnzb, A.n_atoms
+
+    # per-kernel accounting from the events of the timed steps
+    per = {}
+    for ev in timer:
+        kind, epi, e0, e1 = ev
+        key = (kind, epi)
+        per.setdefault(key, [0.0, 0])
+        per[key][0] += e0.elapsed_time(e1)
+        per[key][1] += 1
+    dom = max(per.items(), key=lambda kv: kv[1][0]) if per else None
+    roof = None
+    if dom is not None:
+        (kind, epi), (ms_total, cnt) = dom
+        avg_ms = ms_total / cnt
+        by = kernel_bytes(kind, nnzb, n_atoms, epi)
+        ach = by / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": {"f32": "k_bsr_spmv<float,*,16> (block SpMV, fp32 values)",
+                           "f64": "k_bsr_spmv<double,double,16> (block SpMV, fp64 values)",
+                           "mf64": "k_matfree_epi<double,16> (matrix-free fp64 ANM product)"}[kind],
+                "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // args.steps,
+                "bytes_per_launch": int(by),
+                "share_of_step": round(ms_total / args.steps / 1e3 / sec, 3)}
+    kernel_table = {"%s/%s" % k: {"launches": v[1] // args.steps, "ms_per_step": round(v[0] / args.steps, 3),
+                                  "avg_ms": round(v[0] / v[1], 5)} for k, v in per.items()}
+
+    # other modes, one timed solve each
+    compare = {}
+    for mode in [m for m in args.compare.split(",") if m and m != args.mode]:
+        torch.cuda.synchronize()
+        ta = time.time()
+        _, rr = step(mode)
+        torch.cuda.synchronize()
+        print("bench: compare mode %s done" % mode, file=sys.stderr, flush=True)
+        compare[mode] = {"seconds": round(time.time() - ta, 4), "max_rel_res": float(np.max(rr.rel_res)),
+                         "total_spmv": int(rr.total_spmv),
+                         "max_abs_dlambda_vs_headline": float(np.max(np.abs(rr.eigval - r.eigval)))}
+
+    # achievable bandwidth reference: streaming copy of 2 GiB
+    src = torch.empty(1 << 28, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        ops.stream_copy(src, dst)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        ops.stream_copy(src, dst)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
+    del src, dst
+    if roof is not None:
+        roof["stream_copy_GBs"] = round(copy_gbs, 1)
+        roof["frac_of_stream_copy"] = round(roof["achieved"] / copy_gbs, 4)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.cpu_baseline import cpu_baseline
+        cpu = cpu_baseline(synth.globule(args.cpu_atoms, seed=1), args.atoms, nnzb, args.k, r)
+
+    line = {
+        "metric": "time-to-k-modes at resid\u22641e-8 (s) + SpMV HBM GB/s; mixed/learned vs plain FP64",
+        "value": round(sec, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": {"mixed": "f32 SpMV + f64 Lanczos/RR/polish", "f64": "f64", "mf64": "f64 (matrix-free)"}[args.mode],
+        "data": "synthetic (seeded RSA globule, no PDB input)",
+        "config": {"workload": "globule %d atoms, lowest %d modes, TRLM+Chebyshev, mode=%s" % (args.atoms, args.k, args.mode),
+                   "n_atoms": n_atoms, "dof": 3 * n_atoms, "nnzb_full": nnzb, "k": args.k,
+                   "tol_rel": args.tol, "filter_degree": r.degree, "restarts": r.restarts,
+                   "total_spmv": int(r.total_spmv), "n_spmv": r.n_spmv,
+                   "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
+                   "max_rel_res": float(np.max(r.rel_res)), "lam_min": float(r.eigval[0]),
+                   "lam_max_bound": float(r.lam_max)},
+        "roofline": roof,
+        "kernels": kernel_table,
+        "compare_modes": compare,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -29,6 +29,9 @@ SIGNATURES = {
     "inch_bsr_spmv_part": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp]),
     "inch_bsr_spmm": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp]),
     "inch_anm_matfree_apply": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _vp, _vp, _i32, _vp]),
+    "inch_anm_matfree_spmv": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp,
+                                        _dbl, _vp, _vp, _i32, _vp]),
+    "inch_stream_copy": (_STATUS, [_i64, _vp, _vp, _vp]),
     "inch_multidot_work_size": (_i64, [_i32]),
     "inch_multidot": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
     "inch_gemv_update": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _dbl, _vp, _vp, _vp, _vp]),

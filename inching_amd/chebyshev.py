@@ -239,7 +239,8 @@ class KPMMoments:
     demand; A' depends only on the spectrum bound (cc, dd), not on the wanted interval."""
 
     def __init__(self, A, bound, trials=5, seed=0, precision="f32"):
-        A.ensure(precision)
+        if precision != "mf64":
+            A.ensure(precision)
         self.A = A
         self.precision = precision
         lo, hi = bound
@@ -265,7 +266,10 @@ class KPMMoments:
 
     def _step(self):
         k = len(self.mom)
-        W = self.A.matmat(self.tcur, precision=self.precision)
+        if self.precision == "mf64":
+            W = self.A.matfree(self.tcur)
+        else:
+            W = self.A.matmat(self.tcur, precision=self.precision)
         self.n_spmv += self.tcur.shape[1]
         scal = (1.0 if k == 1 else 2.0) / self.dd
         # device elementwise recurrence on the (n x trials) block
@@ -322,7 +326,8 @@ def optimize_on_memory(A, bound, wanted_left, wanted_count, designated_start=Non
 
 def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
     """(min, max) Ritz values of a 300-step Lanczos run without re-orthogonalisation."""
-    A.ensure(precision)
+    if precision != "mf64":
+        A.ensure(precision)
     n = A.shape[0]
     dev = A.device
     g = torch.Generator(device=dev)
@@ -334,11 +339,14 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
     alpha = torch.zeros(n_tridiag + 1, dtype=torch.float64, device=dev)
     beta = torch.zeros(n_tridiag, dtype=torch.float64, device=dev)
     ws = ops.Workspace(1, dev)
-    vals = A.vals(precision)
+    vals = A.vals(precision) if precision != "mf64" else None
     row = v.view(1, -1)
 
     def av(x, y):
-        ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=A.lanes)
+        if vals is None:
+            ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, x, y)
+        else:
+            ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=A.lanes)
 
     av(v, f)
     ops.multidot(row, 1, f, alpha[0:1], ws)

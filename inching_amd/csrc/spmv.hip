@@ -128,6 +128,9 @@ __global__ void __launch_bounds__(256) k_bsr_spmm(int64_t n, const int64_t *__re
 }
 
 // y_i = plus_i x_i + sum_{j != i, r2 > 0} d (d . (x_i - x_j)) / r2,  d = p_j - p_i
+// (the ANM product recomputed from float64 coordinates: H_ij = -d d^T / r2 and
+// H_ii = -sum_j H_ij). Several right-hand sides: blockIdx.y selects the column of the
+// row-major [3n, nrhs] blocks.
 template <int G>
 __global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__restrict__ xyz,
                                                   const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
@@ -163,6 +166,65 @@ __global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__rest
     const double vi = lane == 0 ? vi0 : (lane == 1 ? vi1 : vi2);
     Y[(3 * row + lane) * nrhs + r] = w + plus_i * vi;
   }
+}
+
+// Single right-hand side with the same fused epilogue as k_bsr_spmv:
+//   out = s1 * (A x - c0 * x) - c2 * z (+ Q term); y = out; acc += mu * out.
+template <typename VT, int G>
+__global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__restrict__ xyz,
+                                                      const int64_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col, double plus_i,
+                                                      const VT *__restrict__ x, VT *__restrict__ y, double s1,
+                                                      double c0, const VT *__restrict__ z, double c2,
+                                                      VT *__restrict__ acc, double mu, const double *__restrict__ Q,
+                                                      const double *__restrict__ qcoef, int mq) {
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  if (row >= n) return;
+  const double pi0 = xyz[3 * row], pi1 = xyz[3 * row + 1], pi2 = xyz[3 * row + 2];
+  const double vi0 = (double)x[3 * row], vi1 = (double)x[3 * row + 1], vi2 = (double)x[3 * row + 2];
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t b = rowptr[row] + lane; b < rowptr[row + 1]; b += G) {
+    const int64_t j = col[b];
+    if (j == row) continue;
+    const double d0 = xyz[3 * j] - pi0, d1 = xyz[3 * j + 1] - pi1, d2 = xyz[3 * j + 2] - pi2;
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (!(r2 > 0.0)) continue;
+    const double t = (d0 * (vi0 - (double)x[3 * j]) + d1 * (vi1 - (double)x[3 * j + 1]) +
+                      d2 * (vi2 - (double)x[3 * j + 2])) / r2;
+    a0 += d0 * t;
+    a1 += d1 * t;
+    a2 += d2 * t;
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, G);
+    a1 += __shfl_xor(a1, off, G);
+    a2 += __shfl_xor(a2, off, G);
+  }
+  if (lane < 3) {
+    const int64_t dof = 3 * row + lane;
+    const double vi = lane == 0 ? vi0 : (lane == 1 ? vi1 : vi2);
+    double out = (lane == 0 ? a0 : (lane == 1 ? a1 : a2)) + plus_i * vi;
+    if (c0 != 0.0) out -= c0 * vi;
+    out *= s1;
+    if (z) out -= c2 * (double)z[dof];
+    if (qcoef) {
+      double q = 0.0;
+      const int64_t nd = 3 * n;
+      for (int m = 0; m < mq; ++m) q += qcoef[m] * Q[m * nd + dof];
+      out += q;
+    }
+    y[dof] = (VT)out;
+    if (acc) acc[dof] = (VT)((double)acc[dof] + mu * out);
+  }
+}
+
+// Streaming copy used to measure the achievable HBM bandwidth (16 B per lane access).
+__global__ void __launch_bounds__(256) k_stream_copy(int64_t n4, const float4 *__restrict__ src,
+                                                      float4 *__restrict__ dst) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------- dense FP64 ---
@@ -402,6 +464,37 @@ inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *
   const int bs = 256;
   dim3 grid((unsigned)((n * G + bs - 1) / bs), (unsigned)nrhs);
   k_matfree<G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, x, y, nrhs);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                  double plus_i, int32_t vdt, const void *x, void *y, double s1, double c0,
+                                  const void *z, double c2, void *acc, double mu, const double *Q,
+                                  const double *qcoef, int32_t m_q, void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && rowptr && col && x && y && x != y, "inch_anm_matfree_spmv: bad args");
+  INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_spmv: qcoef needs Q");
+  hipStream_t st = (hipStream_t)stream;
+  constexpr int G = 16;
+  const int bs = 256;
+  const int64_t grid = (n * G + bs - 1) / bs;
+  if (vdt == INCH_F64)
+    k_matfree_epi<double, G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, (const double *)x, (double *)y, s1,
+                                                  c0, (const double *)z, c2, (double *)acc, mu, Q, qcoef, m_q);
+  else if (vdt == INCH_F32)
+    k_matfree_epi<float, G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, (const float *)x, (float *)y, s1, c0,
+                                                 (const float *)z, c2, (float *)acc, mu, Q, qcoef, m_q);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: bad dtype");
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *stream) {
+  INCH_REQUIRE(bytes > 0 && bytes % 16 == 0 && src && dst, "inch_stream_copy: bytes must be a positive multiple of 16");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n4 = bytes / 16;
+  k_stream_copy<<<grid_for(n4, 256, 8192), 256, 0, st>>>(n4, (const float4 *)src, (float4 *)dst);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

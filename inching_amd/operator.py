@@ -6,8 +6,13 @@
 
 Precision policy: the block values and the Chebyshev recurrence vectors are float32 with
 float32 accumulation in ``precision='f32'`` (mixed) mode and float64 in ``'f64'`` mode;
-v and u (the Lanczos basis side) are always float64. One filtered matvec of degree m is m
-launches of the fused SpMV + three-term-recurrence kernel.
+``'mf64'`` uses no stored values at all: the float64 ANM product is recomputed from the
+float64 coordinates and the neighbour pattern (4 B of index traffic per neighbour instead
+of 36-72 B of block values). v and u (the Lanczos basis side) are always float64. One
+filtered matvec of degree m is m launches of the fused SpMV + three-term-recurrence kernel.
+
+``timer``: optional list; when set, every operator launch is bracketed by HIP events on
+the current stream and the event pairs are appended (kernel-time accounting).
 """
 import torch
 
@@ -15,19 +20,24 @@ from . import ops
 
 
 class Operator:
-    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None):
-        if precision not in ("f32", "f64"):
-            raise ValueError("precision must be 'f32' or 'f64'")
-        A.ensure(precision)
+    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None, timer=None):
+        if precision not in ("f32", "f64", "mf64"):
+            raise ValueError("precision must be 'f32', 'f64' or 'mf64'")
         self.A = A
         self.precision = precision
-        self.vals = A.vals(precision)
+        self.timer = timer
+        if precision == "mf64":
+            self.vals = None
+        else:
+            A.ensure(precision)
+            self.vals = A.vals(precision)
         self.n = A.shape[0]
         self.pol = pol
         self.Q = Q
         self.shift = float(shift)
         self.lanes = lanes or A.lanes
         self.n_spmv = 0
+        self.n_apply = 0
         dev = A.device
         vdt = torch.float32 if precision == "f32" else torch.float64
         self.vdt = vdt
@@ -46,7 +56,18 @@ class Operator:
             self._ws = ops.Workspace(Q.shape[0], dev)
 
     def spmv(self, x, y, **epi):
-        ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.lanes, **epi)
+        if self.timer is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        if self.vals is None:
+            ops.matfree_spmv(self.A.xyz, self.A.rowptr, self.A.col, self.A.plus_i, x, y, **epi)
+        else:
+            ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.lanes, **epi)
+        if self.timer is not None:
+            e1.record()
+            epi = "cheb" if (epi.get("acc") is not None or epi.get("z") is not None) else "plain"
+            self.timer.append((self.precision, epi, e0, e1))
         self.n_spmv += 1
 
     def filtered(self, v64, u64):
@@ -66,6 +87,7 @@ class Operator:
         ops.convert(acc, u64)
 
     def apply(self, v64, u64):
+        self.n_apply += 1
         if self.pol is None:
             self.spmv(v64, u64)
         else:

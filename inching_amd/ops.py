@@ -164,6 +164,34 @@ def matfree_apply(xyz, rowptr, col, plus_i, X, Y, stream=None):
          stream_ptr(stream))
 
 
+def matfree_spmv(xyz, rowptr, col, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None,
+                 qcoef=None, stream=None):
+    """Matrix-free ANM product with the fused epilogue (see bsr_spmv)."""
+    n = xyz.shape[0]
+    nd = 3 * n
+    for t in (x, y, z, acc):
+        if t is not None and t.numel() != nd:
+            raise ValueError("vector length mismatch")
+    vd = dt(x)
+    for t in (y, z, acc):
+        if t is not None and dt(t) != vd:
+            raise ValueError("all vectors must share a dtype")
+    mq = 0
+    if qcoef is not None:
+        if Q is None or Q.dtype != torch.float64 or Q.shape[1] != nd:
+            raise ValueError("Q must be float64 [m, 3n]")
+        mq = Q.shape[0]
+    call("inch_anm_matfree_spmv", n, P(xyz), P(rowptr), P(col), float(plus_i), vd, P(x), P(y), float(s1),
+         float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, stream_ptr(stream))
+
+
+def stream_copy(src, dst, stream=None):
+    nb = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() < nb:
+        raise ValueError("dst too small")
+    call("inch_stream_copy", nb, P(src), P(dst), stream_ptr(stream))
+
+
 class Workspace:
     """Scratch for the two-phase reductions, sized for up to ``J`` dot products."""
 

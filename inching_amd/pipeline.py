@@ -9,9 +9,11 @@ Precision modes:
 * ``"f64"``   float64 block values throughout (the plain FP64 GPU solver);
 * ``"mixed"`` phase 1 runs the float32 operator (values and Chebyshev vectors float32,
               float64 basis / re-orthogonalisation / Rayleigh-Ritz); phase 2 restarts
-              TRLM with the float64 operator from a combination of the phase-1 Ritz
-              vectors (an excellent start vector, so few float64 restarts are needed);
-              a float32 operator alone stalls at ||E|| ~ 6e-8 ||A|| relative residual.
+              TRLM with a float64 operator (``phase2``: 'mf64' matrix-free or 'f64' stored)
+              from a combination of the phase-1 Ritz vectors (an excellent start vector,
+              so few float64 restarts are needed); a float32 operator alone stalls at
+              ||E|| ~ 6e-8 ||A|| relative residual;
+* ``"mf64"``  the matrix-free float64 operator throughout (no stored values).
 Acceptance is always the float64 residual ``||A v - lam v|| / (lam_max ||v||)`` computed
 with the matrix-free float64 operator, after a float64 Rayleigh-Ritz polish.
 """
@@ -59,7 +61,7 @@ def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
 
 def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, bound=None, wanted_right=None,
                  max_degree=5120, min_degree=5, ratio=0.7, hotelling=None, factor=2, maxiter=200, seed=0,
-                 phase1_rel=3e-7, v0=None, verbose=False):
+                 phase1_rel=3e-7, v0=None, verbose=False, phase2="mf64", timer=None):
     """k lowest non-trivial eigenpairs of A = H + PlusI I (RCM order) on the device.
 
     Returns ModeResult with eigval (ascending, includes PlusI), eigvec rows [k, n] (float64
@@ -70,7 +72,7 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
     torch.cuda.synchronize()
     t = time.time()
     if bound is None:
-        lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else "f64")
+        lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision)
         bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
         n_spmv["bound"] = 301
     lam_max = bound[1]
@@ -84,7 +86,7 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
             pol, cnt, interval, nk = CH.optimize_on_memory(
                 A, bound, A.plus_i, wanted_count=max(3 * k, k + 32), designated_start=bound[1] / 10.0,
                 max_degree=max_degree, min_degree=min_degree, margin=max(5, k // 2), trials=5, ratio=ratio,
-                seed=seed, precision="f32" if precision == "mixed" else "f64")
+                seed=seed, precision="f32" if precision == "mixed" else precision)
             n_spmv["kpm"] = nk
         else:
             pol = CH.design_filter(A.plus_i, wanted_right, bound, max_degree, min_degree, ratio)
@@ -94,9 +96,10 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
     check, state = _check_factory(A, Q, lam_max, tol_rel, k)
     start = v0
     total = 0
+    applies = 0
     if precision == "mixed":
         t = time.time()
-        op32 = Operator(A, "f32", pol=pol, Q=Q, shift=hotelling)
+        op32 = Operator(A, "f32", pol=pol, Q=Q, shift=hotelling, timer=timer)
         # phase 1: float32 operator until the float64-verified residual reaches phase1_rel
         # or stops improving (the float32 floor)
         check1, state1 = _check_factory(A, Q, lam_max, phase1_rel, k, stall=0.7)
@@ -105,11 +108,13 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
         phases["phase1_f32"] = time.time() - t
         n_spmv["phase1_f32"] = op32.n_spmv
         total += op32.n_spmv
+        applies += op32.n_apply
         theta1, X1, rel1 = state1["last"]
         if float(np.max(rel1)) <= tol_rel:
             torch.cuda.synchronize()
             return ModeResult(eigval=np.asarray(theta1), eigvec=X1, rel_res=np.asarray(rel1), lam_max=lam_max,
-                              phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r1.iters + 1,
+                              phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r1.iters + 1, n_apply=applies,
+                              ncv=min(max(factor * k, k + 32), A.shape[0] - 1),
                               degree=int(pol.AdjustedDegree) if pol is not None else 0,
                               seconds=time.time() - t_all, precision=precision)
         g = torch.Generator(device=A.device)
@@ -118,15 +123,19 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
         start = (c[:, None] * X1).sum(0)
         del op32, r1
     t = time.time()
-    op64 = Operator(A, "f64", pol=pol, Q=Q, shift=hotelling)
+    p2 = phase2 if precision == "mixed" else precision
+    op64 = Operator(A, p2, pol=pol, Q=Q, shift=hotelling, timer=timer if precision != "mixed" else None)
     r2 = TRLM(op64, k, factor=factor).run(maxiter=maxiter, tol=0.0, v0=start, seed=seed, filtered=filtered,
                                            check=check, verbose=verbose)
-    phases["phase2_f64" if precision == "mixed" else "solve_f64"] = time.time() - t
-    n_spmv["phase2_f64" if precision == "mixed" else "solve_f64"] = op64.n_spmv
+    key = ("phase2_" + p2) if precision == "mixed" else ("solve_" + p2)
+    phases[key] = time.time() - t
+    n_spmv[key] = op64.n_spmv
     total += op64.n_spmv
+    applies += op64.n_apply
     theta, X, rel = state["last"]
     torch.cuda.synchronize()
     return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(rel), lam_max=lam_max,
-                      phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r2.iters + 1,
+                      phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r2.iters + 1, n_apply=applies,
+                      ncv=min(max(factor * k, k + 32), A.shape[0] - 1),
                       degree=int(pol.AdjustedDegree) if pol is not None else 0,
                       seconds=time.time() - t_all, precision=precision)

@@ -132,6 +132,16 @@ inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *
                                    const int32_t *col, double plus_i, const double *x, double *y,
                                    int32_t nrhs, void *stream);
 
+/* Matrix-free ANM product with the fused epilogue of inch_bsr_spmv (vectors float32
+ * or float64 per vdt; geometry and accumulation in float64). */
+inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                  double plus_i, int32_t vdt, const void *x, void *y, double s1, double c0,
+                                  const void *z, double c2, void *acc, double mu, const double *Q,
+                                  const double *qcoef, int32_t m_q, void *stream);
+
+/* dst = src (bytes, multiple of 16): streaming copy for measuring achievable HBM bandwidth. */
+inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *stream);
+
 /* h[j] = sum_i V[j*ld + i] * u[i] for j < J (float64, deterministic two-phase
  * reduction). work: float64 scratch of inch_multidot_work_size(J) elements. */
 int64_t inch_multidot_work_size(int32_t J);
