@@ -1,0 +1,86 @@
+"""Drop-in for ``InchingLiteInteger/Fuel/Coordinate/T1.py`` of the reference: 3DRCM ordering
+and per-row-batch column islands.
+
+``X_KdCuthillMckeeOrder`` (reference :171-323) and ``X_KdUngappedMinMaxNeighbor``
+(reference :331-481) keep their names, arguments and return values. Coordinates are in nm
+and ``rc_Gamma`` in Angstrom as in the reference (the cutoff is divided by 10). Neighbour
+sets come from the device cell list (``inch_neighbors_celllist``); the permutation from the
+host C++ RCM (``inch_rcm_host``), which follows the reference's BFS, child ordering and
+reversal exactly, with ties in the seed order broken by a stable argsort (pass
+``User_SeedOrder`` to reproduce any other tie order, e.g. ``numpy.argsort(degree)``).
+
+Periodic images (``User_DictCharmmGuiPbc``) are not supported by this build.
+"""
+import numpy as np
+import torch
+
+from inching_amd import ops
+from inching_amd.hessian import _to_device_xyz
+
+
+def _neighbours(X, rc_nm):
+    ops.require_gpu()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    xyz = _to_device_xyz(X, dev)
+    ip, ix = ops.neighbours(xyz, rc_nm)
+    return ip.cpu().numpy(), ix.cpu().numpy()
+
+
+def _no_pbc(User_DictCharmmGuiPbc):
+    if User_DictCharmmGuiPbc is not None:
+        raise NotImplementedError("periodic images (User_DictCharmmGuiPbc) are not supported by this build")
+
+
+def X_KdCuthillMckeeOrder(X, rc_Gamma=15.0, Reverse=True, ReturnStat=False, User_maxleafsize=1000,
+                          User_DictCharmmGuiPbc=None, User_SeedOrder=None):
+    """(order, undo) of the 3DRCM permutation; with ReturnStat also mean and std of the
+    neighbour count (self included)."""
+    _no_pbc(User_DictCharmmGuiPbc)
+    ip, ix = _neighbours(X, rc_Gamma / 10.0)
+    order, undo = ops.rcm_host(ip, ix, seed_order=User_SeedOrder, reverse=Reverse)
+    if ReturnStat:
+        deg = np.diff(ip)
+        return order, undo, np.mean(deg), np.std(deg)
+    return order, undo
+
+
+def ungapped_islands(indptr, indices, n_atoms, maxleafsize=1000, gap_size=100):
+    """Per batch the sorted unique neighbour columns split at gaps wider than ``gap_size``,
+    with the reference's exceptions (the first five and the fifth-last difference are never
+    gaps; a gap whose left island reaches the batch's diagonal block is not cut)."""
+    head = ops.batch_head(n_atoms, maxleafsize)
+    out = {}
+    total = 0
+    for b in range(len(head) - 1):
+        start, end = int(head[b]), int(head[b + 1])
+        cols = indices[indptr[start]:indptr[end]]
+        total += cols.size
+        uniq = np.unique(cols)
+        nu = uniq.size
+        cand = np.nonzero(np.diff(uniq) > gap_size)[0]
+        gse = [int(uniq[0])]
+        height = end - start
+        for iii in cand:
+            if iii == nu - 5 or iii < 5:
+                continue
+            if (gse[-1] + 5 >= start) and (int(uniq[iii]) + 1 <= start + height + 5):
+                continue
+            gse.extend([int(uniq[iii]) + 1, int(uniq[iii + 1])])
+        gse.append(int(uniq[-1]) + 1)
+        out[b] = [(gse[t], gse[t + 1]) for t in range(0, len(gse), 2)]
+    half_nnz = (((total - n_atoms) / 2) + n_atoms) * 9
+    return out, half_nnz
+
+
+def X_KdUngappedMinMaxNeighbor(X, rc_Gamma=15.0, maxleafsize=1000, CollectStat=False, User_ReturnHalfNnz=False,
+                               User_GapSize=100, User_DictCharmmGuiPbc=None, SliceForm=True):
+    """{batch: [(start, end), ...]} column islands (slice form) [, HalfNnz]."""
+    _no_pbc(User_DictCharmmGuiPbc)
+    ip, ix = _neighbours(X, rc_Gamma / 10.0)
+    n = ip.shape[0] - 1
+    islands, half = ungapped_islands(ip, ix, n, maxleafsize=maxleafsize, gap_size=User_GapSize)
+    if not SliceForm:
+        islands = {b: [(s, e - 1) for (s, e) in v] for b, v in islands.items()}
+    if User_ReturnHalfNnz:
+        return islands, half
+    return islands

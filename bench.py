@@ -36,12 +36,56 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--atoms", type=int, default=300000)
     ap.add_argument("--k", type=int, default=64)
-    ap.add_argument("--mode", default="mixed", help="headline operator mode: mixed | mf64 | f64")
-    ap.add_argument("--compare", default="f64,mf64", help="other modes timed once each")
+    ap.add_argument("--mode", default="mf64", help="headline operator mode: mixed | mf64 | f64")
+    ap.add_argument("--compare", default="f64,mixed", help="other modes timed once each")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
+    ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
+                    help="only build the matrix and launch each operator kernel REPS times (profiling runs)")
     return ap.parse_args()
+
+
+def spmv_only(args):
+    """Short program for counter collection: the operator kernels of one solve, launched
+    REPS times each on the workload's matrix, plus the streaming-copy reference kernel."""
+    import torch
+
+    from inching_amd import ops, synth
+    from inching_amd.hessian import build_hessian
+    X = synth.globule(args.atoms, seed=1)
+    A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=("f32", "f64"))
+    n = A.shape[0]
+    out = {"nnzb": A.nnzb, "n_atoms": A.n_atoms}
+    x32 = torch.randn(n, dtype=torch.float32, device="cuda")
+    z32, y32, acc32 = torch.randn_like(x32), torch.empty_like(x32), torch.zeros_like(x32)
+    x64 = torch.randn(n, dtype=torch.float64, device="cuda")
+    z64, y64, acc64 = torch.randn_like(x64), torch.empty_like(x64), torch.zeros_like(x64)
+    cases = {
+        "bsr_f32_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x32, y32, s1=0.1, c0=2.0, z=z32,
+                                             c2=1.0, acc=acc32, mu=0.01, lanes=A.lanes),
+        "bsr_f64_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, x64, y64, s1=0.1, c0=2.0, z=z64,
+                                             c2=1.0, acc=acc64, mu=0.01, lanes=A.lanes),
+        "matfree_f64_cheb": lambda: ops.matfree_spmv(A.xyz, A.rowptr, A.col, 1.0, x64, y64, s1=0.1, c0=2.0, z=z64,
+                                                     c2=1.0, acc=acc64, mu=0.01),
+    }
+    for name, fn in cases.items():
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.spmv_only):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name + "_avg_ms"] = round(e0.elapsed_time(e1) / args.spmv_only, 5)
+    src = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.spmv_only):
+        ops.stream_copy(src, dst)
+    torch.cuda.synchronize()
+    out["stream_copy_bytes_read"] = src.numel() * 8
+    print(json.dumps(out))
 
 
 def kernel_bytes(kind, nnzb, n_atoms, epi):
@@ -62,6 +106,8 @@ def main():
         print("bench.py: this build measures one GPU; --gpus %d is not supported (single-GPU path, "
               "see DESIGN.md for how the solve would shard)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.spmv_only:
+        return spmv_only(args)
     import numpy as np
     import torch
 
@@ -90,11 +136,10 @@ def main():
         step(args.mode)
         print("bench: warmup %d done" % w, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    timer = []
     results = []
     t0 = time.time()
     for s_ in range(args.steps):
-        results.append(step(args.mode, timer=timer))
+        results.append(step(args.mode))
         print("bench: step %d done" % s_, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t1 = time.time()
@@ -102,34 +147,46 @@ def main():
     A, r = results[-1]
     nnzb, n_atoms = A.nnzb, A.n_atoms
 
-    # per-kernel accounting from the events of the timed steps
-    per = {}
-    for ev in timer:
-        kind, epi, e0, e1 = ev
-        key = (kind, epi)
-        per.setdefault(key, [0.0, 0])
-        per[key][0] += e0.elapsed_time(e1)
-        per[key][1] += 1
-    dom = max(per.items(), key=lambda kv: kv[1][0]) if per else None
-    roof = None
-    if dom is not None:
-        (kind, epi), (ms_total, cnt) = dom
+    def kernel_stats(timer, steps):
+        """Per-kernel totals from HIP event pairs around every operator launch."""
+        per = {}
+        for kind, epi, e0, e1 in timer:
+            per.setdefault((kind, epi), [0.0, 0])
+            per[(kind, epi)][0] += e0.elapsed_time(e1)
+            per[(kind, epi)][1] += 1
+        return per
+
+    def roofline(per, step_s, steps):
+        if not per:
+            return None
+        (kind, epi), (ms_total, cnt) = max(per.items(), key=lambda kv: kv[1][0])
         avg_ms = ms_total / cnt
         by = kernel_bytes(kind, nnzb, n_atoms, epi)
         ach = by / (avg_ms * 1e-3) / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": {"f32": "k_bsr_spmv<float,*,16> (block SpMV, fp32 values)",
-                           "f64": "k_bsr_spmv<double,double,16> (block SpMV, fp64 values)",
-                           "mf64": "k_matfree_epi<double,16> (matrix-free fp64 ANM product)"}[kind],
-                "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // args.steps,
-                "bytes_per_launch": int(by),
-                "share_of_step": round(ms_total / args.steps / 1e3 / sec, 3)}
-    kernel_table = {"%s/%s" % k: {"launches": v[1] // args.steps, "ms_per_step": round(v[0] / args.steps, 3),
+                "kernel": {"f32": "k_bsr_spmv<float,float,16> (block SpMV, fp32 values, Chebyshev epilogue)",
+                           "f64": "k_bsr_spmv<double,double,16> (block SpMV, fp64 values, Chebyshev epilogue)",
+                           "mf64": "k_matfree_epi<double,16> (matrix-free fp64 ANM product, Chebyshev epilogue)"}[kind],
+                "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // steps,
+                "bytes_per_launch": int(by), "kernel_share_of_step": round(ms_total / steps / 1e3 / step_s, 3)}
+
+    # instrumented solve of the same workload: HIP events around every operator launch (the
+    # timed steps replay captured HIP graphs, where per-launch events cannot be recorded)
+    timer = []
+    torch.cuda.synchronize()
+    ti = time.time()
+    step(args.mode, timer=timer)
+    torch.cuda.synchronize()
+    t_instr = time.time() - ti
+    per = kernel_stats(timer, 1)
+    roof = roofline(per, t_instr, 1)
+    kernel_table = {"%s/%s" % k: {"launches": v[1], "ms_per_step": round(v[0], 3),
                                   "avg_ms": round(v[0] / v[1], 5)} for k, v in per.items()}
 
-    # other modes, one timed solve each
+    # other modes, one timed solve each (graphs on), plus an instrumented block-SpMV solve
     compare = {}
+    roof_block = None
     for mode in [m for m in args.compare.split(",") if m and m != args.mode]:
         torch.cuda.synchronize()
         ta = time.time()
@@ -139,6 +196,14 @@ def main():
         compare[mode] = {"seconds": round(time.time() - ta, 4), "max_rel_res": float(np.max(rr.rel_res)),
                          "total_spmv": int(rr.total_spmv),
                          "max_abs_dlambda_vs_headline": float(np.max(np.abs(rr.eigval - r.eigval)))}
+        if mode == "mixed":
+            tb = []
+            torch.cuda.synchronize()
+            tt = time.time()
+            step(mode, timer=tb)
+            torch.cuda.synchronize()
+            pb = {k: v for k, v in kernel_stats(tb, 1).items() if k[0] == "f32"}
+            roof_block = roofline(pb, time.time() - tt, 1)
 
     # achievable bandwidth reference: streaming copy of 2 GiB
     src = torch.empty(1 << 28, dtype=torch.float64, device=dev)
@@ -153,9 +218,10 @@ def main():
     torch.cuda.synchronize()
     copy_gbs = 2 * src.numel() * 8 / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
     del src, dst
-    if roof is not None:
-        roof["stream_copy_GBs"] = round(copy_gbs, 1)
-        roof["frac_of_stream_copy"] = round(roof["achieved"] / copy_gbs, 4)
+    for rf in (roof, roof_block):
+        if rf is not None:
+            rf["stream_copy_GBs"] = round(copy_gbs, 1)
+            rf["frac_of_stream_copy"] = round(rf["achieved"] / copy_gbs, 4)
 
     cpu = None
     if not args.no_cpu:
@@ -173,7 +239,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": {"mixed": "f32 SpMV + f64 Lanczos/RR/polish", "f64": "f64", "mf64": "f64 (matrix-free)"}[args.mode],
+        "dtype": {"mixed": "f32", "f64": "f64", "mf64": "f64"}[args.mode],
         "data": "synthetic (seeded RSA globule, no PDB input)",
         "config": {"workload": "globule %d atoms, lowest %d modes, TRLM+Chebyshev, mode=%s" % (args.atoms, args.k, args.mode),
                    "n_atoms": n_atoms, "dof": 3 * n_atoms, "nnzb_full": nnzb, "k": args.k,
@@ -181,8 +247,10 @@ def main():
                    "total_spmv": int(r.total_spmv), "n_spmv": r.n_spmv,
                    "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
                    "max_rel_res": float(np.max(r.rel_res)), "lam_min": float(r.eigval[0]),
+                   "speedup_vs_plain_f64": (round(compare["f64"]["seconds"] / sec, 3) if "f64" in compare else None),
                    "lam_max_bound": float(r.lam_max)},
         "roofline": roof,
+        "roofline_block_spmv_f32": roof_block,
         "kernels": kernel_table,
         "compare_modes": compare,
         "cpu_baseline": cpu,

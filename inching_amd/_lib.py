@@ -39,6 +39,11 @@ SIGNATURES = {
     "inch_vec_axpy_dev": (_STATUS, [_i64, _vp, _dbl, _vp, _vp, _vp]),
     "inch_vec_convert": (_STATUS, [_i64, _vp, _i32, _vp, _i32, _vp]),
     "inch_vec_axpby": (_STATUS, [_i64, _dbl, _vp, _i32, _dbl, _vp, _i32, _vp]),
+    "inch_gram_work_size": (_i64, [_i32, _i32]),
+    "inch_gram": (_STATUS, [_i64, _i32, _vp, _i64, _i32, _vp, _i64, _vp, _vp, _vp]),
+    "inch_residual_norms": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "inch_cgs2_work_size": (_i64, [_i32]),
+    "inch_cgs2_reorth": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "inch_rr_combine": (_STATUS, [_i64, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
 }
 

@@ -127,44 +127,71 @@ __global__ void __launch_bounds__(256) k_bsr_spmm(int64_t n, const int64_t *__re
   }
 }
 
+// 1/r2 in float64 from the float32 hardware reciprocal and two Newton steps (relative
+// error of the seed ~1e-7, squared twice: below float64 rounding); r2 is a squared
+// inter-atomic distance, never tiny or huge, so no range reduction is needed.
+__device__ __forceinline__ double recip_r2(double r2) {
+  double inv = (double)__builtin_amdgcn_rcpf((float)r2);
+  inv = inv * (2.0 - r2 * inv);
+  inv = inv * (2.0 - r2 * inv);
+  return inv;
+}
+
 // y_i = plus_i x_i + sum_{j != i, r2 > 0} d (d . (x_i - x_j)) / r2,  d = p_j - p_i
 // (the ANM product recomputed from float64 coordinates: H_ij = -d d^T / r2 and
-// H_ii = -sum_j H_ij). Several right-hand sides: blockIdx.y selects the column of the
-// row-major [3n, nrhs] blocks.
-template <int G>
+// H_ii = -sum_j H_ij). Several right-hand sides, row-major [3n, nrhs]: blockIdx.y picks
+// a chunk of R columns; the geometry of a neighbour is computed once per chunk.
+template <int G, int R>
 __global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__restrict__ xyz,
                                                   const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                                                   double plus_i, const double *__restrict__ X, double *__restrict__ Y,
                                                   int nrhs) {
   const int lane = threadIdx.x & (G - 1);
   const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
-  const int r = blockIdx.y;
+  const int r0 = blockIdx.y * R;
   if (row >= n) return;
   const double pi0 = xyz[3 * row], pi1 = xyz[3 * row + 1], pi2 = xyz[3 * row + 2];
-  const double vi0 = X[(3 * row) * nrhs + r], vi1 = X[(3 * row + 1) * nrhs + r], vi2 = X[(3 * row + 2) * nrhs + r];
-  double a0 = 0, a1 = 0, a2 = 0;
+  double vi[3][R], a[3][R];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      vi[c][r] = (r0 + r < nrhs) ? X[(3 * row + c) * nrhs + r0 + r] : 0.0;
+      a[c][r] = 0.0;
+    }
   for (int64_t b = rowptr[row] + lane; b < rowptr[row + 1]; b += G) {
     const int64_t j = col[b];
     if (j == row) continue;
     const double d0 = xyz[3 * j] - pi0, d1 = xyz[3 * j + 1] - pi1, d2 = xyz[3 * j + 2] - pi2;
     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
     if (!(r2 > 0.0)) continue;
-    const double t = (d0 * (vi0 - X[(3 * j) * nrhs + r]) + d1 * (vi1 - X[(3 * j + 1) * nrhs + r]) +
-                      d2 * (vi2 - X[(3 * j + 2) * nrhs + r])) / r2;
-    a0 += d0 * t;
-    a1 += d1 * t;
-    a2 += d2 * t;
+    const double inv = recip_r2(r2);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r0 + r < nrhs) {
+        const double t = (d0 * (vi[0][r] - X[(3 * j) * nrhs + r0 + r]) + d1 * (vi[1][r] - X[(3 * j + 1) * nrhs + r0 + r]) +
+                          d2 * (vi[2][r] - X[(3 * j + 2) * nrhs + r0 + r])) * inv;
+        a[0][r] += d0 * t;
+        a[1][r] += d1 * t;
+        a[2][r] += d2 * t;
+      }
+    }
   }
 #pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
-    a0 += __shfl_xor(a0, off, G);
-    a1 += __shfl_xor(a1, off, G);
-    a2 += __shfl_xor(a2, off, G);
-  }
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int r = 0; r < R; ++r) a[c][r] += __shfl_xor(a[c][r], off, G);
   if (lane < 3) {
-    const double w = lane == 0 ? a0 : (lane == 1 ? a1 : a2);
-    const double vi = lane == 0 ? vi0 : (lane == 1 ? vi1 : vi2);
-    Y[(3 * row + lane) * nrhs + r] = w + plus_i * vi;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (r0 + r < nrhs) {
+        const double w = lane == 0 ? a[0][r] : (lane == 1 ? a[1][r] : a[2][r]);
+        const double v = lane == 0 ? vi[0][r] : (lane == 1 ? vi[1][r] : vi[2][r]);
+        Y[(3 * row + lane) * nrhs + r0 + r] = w + plus_i * v;
+      }
+    }
   }
 }
 
@@ -191,7 +218,7 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
     const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
     if (!(r2 > 0.0)) continue;
     const double t = (d0 * (vi0 - (double)x[3 * j]) + d1 * (vi1 - (double)x[3 * j + 1]) +
-                      d2 * (vi2 - (double)x[3 * j + 2])) / r2;
+                      d2 * (vi2 - (double)x[3 * j + 2])) * recip_r2(r2);
     a0 += d0 * t;
     a1 += d1 * t;
     a2 += d2 * t;
@@ -258,13 +285,23 @@ __global__ void __launch_bounds__(256) k_multidot_part(int64_t len, int J, const
   }
 }
 
-__global__ void k_reduce_parts(int nparts, int J, const double *__restrict__ part, double *__restrict__ h,
-                               int do_sqrt) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// Final stage of the two-phase reductions: one 256-thread workgroup per output j sums
+// the nparts partials part[p*J + j] with a fixed strided order and a fixed tree
+// (deterministic, and no serial chain of nparts dependent loads).
+__global__ void __launch_bounds__(256) k_reduce_parts(int nparts, int J, const double *__restrict__ part,
+                                                      double *__restrict__ h, int do_sqrt) {
+  const int j = blockIdx.x;
   if (j >= J) return;
   double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += part[(int64_t)p * J + j];
-  h[j] = do_sqrt ? sqrt(s) : s;
+  for (int p = threadIdx.x; p < nparts; p += blockDim.x) s += part[(int64_t)p * J + j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (red[0] + red[1]) + (red[2] + red[3]);
+    h[j] = do_sqrt ? sqrt(t) : t;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_gemv_update(int64_t len, int J, const double *__restrict__ V, int64_t ld,
@@ -460,10 +497,10 @@ inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *
   INCH_REQUIRE(n > 0 && xyz && rowptr && col && x && y && nrhs > 0 && x != y, "inch_anm_matfree_apply: bad args");
   INCH_REQUIRE(nrhs <= 65535, "inch_anm_matfree_apply: nrhs too large");
   hipStream_t st = (hipStream_t)stream;
-  constexpr int G = 16;
+  constexpr int G = 16, R = 8;
   const int bs = 256;
-  dim3 grid((unsigned)((n * G + bs - 1) / bs), (unsigned)nrhs);
-  k_matfree<G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, x, y, nrhs);
+  dim3 grid((unsigned)((n * G + bs - 1) / bs), (unsigned)((nrhs + R - 1) / R));
+  k_matfree<G, R><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, x, y, nrhs);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
@@ -510,7 +547,7 @@ inch_status inch_multidot(int64_t len, int32_t J, const double *V, int64_t ld, c
   dim3 grid(gx, (J + JT - 1) / JT);
   k_multidot_part<JT><<<grid, 256, 0, st>>>(len, J, V, ld, u, work);
   HIP_TRY(hipGetLastError());
-  k_reduce_parts<<<(J + 255) / 256, 256, 0, st>>>(gx, J, work, h, 0);
+  k_reduce_parts<<<J, 256, 0, st>>>(gx, J, work, h, 0);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
@@ -526,7 +563,7 @@ inch_status inch_gemv_update(int64_t len, int32_t J, const double *V, int64_t ld
                                                                            nrm2_out ? work : nullptr);
   HIP_TRY(hipGetLastError());
   if (nrm2_out) {
-    k_reduce_parts<<<1, 256, 0, st>>>(gx, 1, work, nrm2_out, 1);
+    k_reduce_parts<<<1, 256, 0, st>>>(gx, 1, work, nrm2_out, 1);  // one workgroup, one output
     HIP_TRY(hipGetLastError());
   }
   return INCH_OK;

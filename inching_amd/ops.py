@@ -248,3 +248,62 @@ def rr_combine(V, m, W, out, stream=None):
         raise ValueError("rr_combine output must not alias V")
     call("inch_rr_combine", V.shape[1], int(m), int(k), P(V), V.stride(0), P(W.contiguous()), P(out),
          out.stride(0), stream_ptr(stream))
+
+
+class GramWorkspace:
+    def __init__(self, k1, k2, device):
+        from ._lib import lib
+        self.k1, self.k2 = int(k1), int(k2)
+        size = max(int(lib().inch_gram_work_size(self.k1, self.k2)), 256 * max(self.k1, self.k2))
+        self.buf = torch.empty(size, dtype=torch.float64, device=device)
+
+
+def gram(X, Y, G=None, ws=None, stream=None):
+    """G = X Y^T for float64 row blocks X [k1, n], Y [k2, n] (MFMA kernel)."""
+    k1, k2 = X.shape[0], Y.shape[0]
+    if X.shape[1] != Y.shape[1] or X.dtype != torch.float64 or Y.dtype != torch.float64:
+        raise ValueError("gram: X and Y must be float64 with equal row length")
+    if X.stride(1) != 1 or Y.stride(1) != 1:
+        raise ValueError("gram: rows must be contiguous")
+    if ws is None or ws.k1 < k1 or ws.k2 < k2 or ws.k1 * ws.k2 < k1 * k2:
+        ws = GramWorkspace(k1, k2, X.device)
+    if G is None:
+        G = torch.empty((k1, k2), dtype=torch.float64, device=X.device)
+    call("inch_gram", X.shape[1], k1, P(X), X.stride(0), k2, P(Y), Y.stride(0), P(G), P(ws.buf), stream_ptr(stream))
+    return G
+
+
+def residual_norms(X, Y, theta, out=None, ws=None, stream=None):
+    """out[j] = ||Y[j] - theta[j] X[j]|| for float64 row blocks."""
+    k = X.shape[0]
+    if Y.shape != X.shape or theta.numel() < k:
+        raise ValueError("residual_norms: shape mismatch")
+    if ws is None or ws.buf.numel() < 256 * k:
+        ws = GramWorkspace(k, 1, X.device)
+    if out is None:
+        out = torch.empty(k, dtype=torch.float64, device=X.device)
+    call("inch_residual_norms", X.shape[1], k, P(X), X.stride(0), P(Y), Y.stride(0), P(theta), P(out), P(ws.buf),
+         stream_ptr(stream))
+    return out
+
+
+CGS2_MAX_J = 256
+
+
+class CGS2Workspace:
+    def __init__(self, J, device):
+        from ._lib import lib
+        self.J = int(J)
+        self.buf = torch.empty(int(lib().inch_cgs2_work_size(self.J)), dtype=torch.float64, device=device)
+
+
+def cgs2_reorth(V, J, h1, u, beta_out, vnext=None, ws=None, stream=None):
+    """u' = u - V[:J]^T h1; beta = ||u'||; u = u' - V[:J]^T (V[:J] u'); vnext = u / beta."""
+    if J > CGS2_MAX_J or J < 1:
+        raise ValueError("cgs2_reorth: J must be in 1..256")
+    if V.shape[0] < J or V.shape[1] != u.numel() or h1.numel() < J:
+        raise ValueError("cgs2_reorth: shape mismatch")
+    if ws is None or ws.J < J:
+        ws = CGS2Workspace(J, u.device)
+    call("inch_cgs2_reorth", u.numel(), int(J), P(V), V.stride(0), P(h1), P(u), P(vnext), P(beta_out), P(ws.buf),
+         stream_ptr(stream))

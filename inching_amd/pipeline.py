@@ -40,10 +40,16 @@ def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
     """TRLM callback: float64 RR polish of the current Ritz block; stop when all k pairs
     meet ``tol_rel`` or (``stall`` given) when the worst residual improves by less than
     that factor between restarts."""
-    state = {"calls": 0, "last": None, "hist": []}
+    state = {"calls": 0, "last": None, "hist": [], "armed": False}
 
     def check(trlm, S):
         state["calls"] += 1
+        # the float64 check costs a k-vector polish; run it once the Lanczos residual
+        # estimate is within reach of the target (then every restart), else every 4th
+        gate = 1e-5 * np.sqrt(k) * max(1.0, float(np.max(np.abs(S))))
+        if not state["armed"] and trlm.last_res > gate and state["calls"] % 4 != 0:
+            return False
+        state["armed"] = True
         theta, X, info = polish(A, trlm.V[:k], Q=Q, lam_max=lam_max)
         rel = info["res"] / lam_max
         worst = float(np.max(rel))

@@ -14,16 +14,13 @@ import torch
 from . import ops
 
 
-def _gram(X, Y, ws):
-    """G[a, b] = X[a] . Y[b] for row blocks X [k1, n], Y [k2, n] (float64 device)."""
-    k1, k2 = X.shape[0], Y.shape[0]
-    G = torch.empty((k2, k1), dtype=torch.float64, device=X.device)
-    for b in range(k2):
-        ops.multidot(X, k1, Y[b], G[b], ws)
-    return G.T.cpu().numpy()
+def _gram(X, Y, ws=None):
+    """G[a, b] = X[a] . Y[b] for row blocks X [k1, n], Y [k2, n] (float64 MFMA kernel) -> numpy."""
+    return ops.gram(X, Y, ws=ws).cpu().numpy()
 
 
 def _combine(X, W):
+    """out[q] = sum_r W[r, q] X[r] (W numpy [m, k])."""
     out = torch.empty((W.shape[1], X.shape[1]), dtype=torch.float64, device=X.device)
     ops.rr_combine(X, W.shape[0], torch.as_tensor(np.ascontiguousarray(W), device=X.device), out)
     return out
@@ -32,13 +29,14 @@ def _combine(X, W):
 def orthonormalise(X, Q=None, passes=2):
     """Rows of X made orthonormal (and orthogonal to the rows of Q when given)."""
     k = X.shape[0]
-    ws = ops.Workspace(max(k, 1 if Q is None else Q.shape[0]), X.device)
+    m = 0 if Q is None else Q.shape[0]
+    ws = ops.GramWorkspace(max(k, m, 1), max(k, m, 1), X.device)
     if Q is not None:
-        h = torch.empty(Q.shape[0], dtype=torch.float64, device=X.device)
         for _ in range(2):
-            for j in range(k):
-                ops.multidot(Q, Q.shape[0], X[j], h, ws)
-                ops.gemv_update(Q, Q.shape[0], h, X[j], 1.0)
+            C = ops.gram(X, Q, ws=ws)                     # [k, m]
+            P = torch.empty_like(X)
+            ops.rr_combine(Q, m, C.T.contiguous(), P)     # P[q] = sum_r C[q, r] Q[r]
+            ops.axpby(-1.0, P.view(-1), 1.0, X.view(-1))
     for _ in range(passes):
         G = _gram(X, X, ws)
         G = 0.5 * (G + G.T)
@@ -58,16 +56,9 @@ def apply_fp64(A, X, operator="matfree"):
 
 
 def residual_norms(X, Y, theta):
-    """||Y[j] - theta_j X[j]||_2 for each row j."""
-    k = X.shape[0]
-    ws = ops.Workspace(1, X.device)
+    """||Y[j] - theta_j X[j]||_2 for each row j (one fused kernel)."""
     th = torch.as_tensor(np.asarray(theta, dtype=np.float64), device=X.device)
-    out = torch.empty(k, dtype=torch.float64, device=X.device)
-    tmp = torch.empty(X.shape[1], dtype=torch.float64, device=X.device)
-    for j in range(k):
-        tmp.copy_(Y[j])
-        ops.gemv_update(X[j:j + 1], 1, th[j:j + 1], tmp, 1.0, nrm2_out=out[j:j + 1], ws=ws)
-    return out.cpu().numpy()
+    return ops.residual_norms(X, Y, th).cpu().numpy()
 
 
 def polish(A, Vrows, Q=None, operator="matfree", lam_max=None):
@@ -75,8 +66,7 @@ def polish(A, Vrows, Q=None, operator="matfree", lam_max=None):
     eigenvectors as rows [k, n], info dict with absolute and relative residuals)."""
     X = orthonormalise(Vrows.to(torch.float64).contiguous().clone(), Q=Q)
     Y = apply_fp64(A, X, operator)
-    ws = ops.Workspace(X.shape[0], X.device)
-    Hm = _gram(X, Y, ws)
+    Hm = _gram(X, Y)
     Hm = 0.5 * (Hm + Hm.T)
     theta, C = np.linalg.eigh(Hm)
     X = _combine(X, C)

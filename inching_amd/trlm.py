@@ -44,9 +44,14 @@ class TRLMResult:
 
 
 class TRLM:
-    """Device TRLM on an ``Operator`` (see operator.py)."""
+    """Device TRLM on an ``Operator`` (see operator.py).
 
-    def __init__(self, op, k, factor=2, ncv=None):
+    ``use_graph``: the restart body (restart vector, first extension step and the Lanczos
+    sweep k+1..ncv-1) launches the same kernels on the same buffers every restart, so it
+    is captured once into a HIP graph and replayed; only beta_k is written from the host
+    before each replay."""
+
+    def __init__(self, op, k, factor=2, ncv=None, use_graph=True):
         n = op.n
         if k >= n:
             raise AssertionError("ABORTED. k must be smaller than n")
@@ -64,6 +69,10 @@ class TRLM:
         self.scal = torch.empty(1, dtype=torch.float64, device=dev)
         self.betak_d = torch.empty(self.k, dtype=torch.float64, device=dev)
         self.ws = ops.Workspace(self.ncv + 1, dev)
+        self.cgs_ws = ops.CGS2Workspace(min(self.ncv, ops.CGS2_MAX_J), dev)
+        self.use_graph = bool(use_graph) and op.timer is None
+        self.graph = None
+        self._graph_counts = (0, 0)
 
     # -- Lanczos steps i_start .. i_end-1 (the last step leaves its residual in u)
     def lanczos(self, i_start, i_end):
@@ -72,12 +81,49 @@ class TRLM:
             self.op.apply(V[i], u)
             ops.multidot(V, i + 1, u, h, ws)
             self.alpha[i:i + 1].copy_(h[i:i + 1])
+            last = i >= i_end - 1
+            if i + 1 <= ops.CGS2_MAX_J:
+                # fused: update + second-pass dots in one read of V, then update + normalise
+                ops.cgs2_reorth(V, i + 1, h, u, self.beta[i:i + 1], vnext=None if last else V[i + 1],
+                                ws=self.cgs_ws)
+                continue
             ops.gemv_update(V, i + 1, h, u, 1.0, nrm2_out=self.beta[i:i + 1], ws=ws)
             ops.multidot(V, i + 1, u, h, ws)
             ops.gemv_update(V, i + 1, h, u, 1.0)
-            if i >= i_end - 1:
+            if last:
                 break
             ops.scale_div(u, self.beta[i:i + 1], V[i + 1])
+
+    def _body(self):
+        """Restart vector (orthogonalised against the k Ritz vectors), the arrowhead
+        extension step and the Lanczos sweep k+1 .. ncv-1."""
+        k, ncv = self.k, self.ncv
+        V, u, ws = self.V, self.u, self.ws
+        ops.multidot(V, k, u, self.h, ws)
+        ops.gemv_update(V, k, self.h, u, 1.0, nrm2_out=self.scal, ws=ws)
+        ops.scale_div(u, self.scal, V[k])
+        self.op.apply(V[k], u)
+        ops.multidot(V[k:k + 1], 1, u, self.alpha[k:k + 1], ws)
+        ops.axpy_dev(self.alpha[k:k + 1], -1.0, V[k], u)
+        ops.gemv_update(V, k, self.betak_d, u, 1.0, nrm2_out=self.beta[k:k + 1], ws=ws)
+        ops.scale_div(u, self.beta[k:k + 1], V[k + 1])
+        self.lanczos(k + 1, ncv)
+
+    def _run_body(self):
+        if not self.use_graph:
+            self._body()
+            return
+        if self.graph is None:
+            n0, a0 = self.op.n_spmv, self.op.n_apply
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._body()
+            self.graph = g
+            self._graph_counts = (self.op.n_spmv - n0, self.op.n_apply - a0)
+        else:
+            self.op.n_spmv += self._graph_counts[0]
+            self.op.n_apply += self._graph_counts[1]
+        self.graph.replay()
 
     def _restart_rr(self, alpha_h, beta_h, beta_k, filtered):
         S, W = select_ritz(alpha_h, beta_h, beta_k, self.k, filtered)
@@ -109,29 +155,21 @@ class TRLM:
         history = [res]
         it = 0
         stop = False
+        self.last_res = res
         if check is not None:
             stop = bool(check(self, S))
         for it in range(maxiter):
             if res <= tol or stop:
                 break
-            V, u, ws = self.V, self.u, self.ws
-            # restart vector: u orthogonalised against the k Ritz vectors
-            ops.multidot(V, k, u, self.h, ws)
-            ops.gemv_update(V, k, self.h, u, 1.0, nrm2_out=self.scal, ws=ws)
-            ops.scale_div(u, self.scal, V[k])
-            self.op.apply(V[k], u)
-            ops.multidot(V[k:k + 1], 1, u, self.alpha[k:k + 1], ws)
-            ops.axpy_dev(self.alpha[k:k + 1], -1.0, V[k], u)
-            self.betak_d.copy_(torch.as_tensor(beta_k, device=V.device))
-            ops.gemv_update(V, k, self.betak_d, u, 1.0, nrm2_out=self.beta[k:k + 1], ws=ws)
-            ops.scale_div(u, self.beta[k:k + 1], V[k + 1])
-            self.lanczos(k + 1, ncv)
+            self.betak_d.copy_(torch.as_tensor(beta_k, device=self.V.device))
+            self._run_body()
             alpha_h = self.alpha.cpu().numpy().copy()
             beta_h = self.beta.cpu().numpy().copy()
             alpha_h[:k] = S
             beta_h[:k] = 0.0
             S, beta_k, res = self._restart_rr(alpha_h, beta_h, beta_k, filtered)
             history.append(res)
+            self.last_res = res
             if verbose:
                 print("TRLM restart %d  ||beta_k|| = %.3e  spmv = %d" % (it, res, self.op.n_spmv))
             if check is not None:

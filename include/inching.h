@@ -166,6 +166,26 @@ inch_status inch_vec_axpby(int64_t len, double a, const void *x, int32_t xdt, do
 inch_status inch_vec_convert(int64_t len, const void *src, int32_t sdt, void *dst, int32_t ddt,
                              void *stream);
 
+/* G[a*k2 + b] = sum_i X[a*ldx + i] * Y[b*ldy + i]  (a < k1, b < k2; float64 MFMA
+ * 16x16x4 tiles, deterministic two-phase reduction). work: inch_gram_work_size(k1, k2)
+ * doubles. */
+int64_t inch_gram_work_size(int32_t k1, int32_t k2);
+inch_status inch_gram(int64_t len, int32_t k1, const double *X, int64_t ldx, int32_t k2, const double *Y,
+                      int64_t ldy, double *G, double *work, void *stream);
+
+/* out[j] = || Y[j*ldy + :] - theta[j] * X[j*ldx + :] ||_2 for j < k (work >= 256*k doubles). */
+inch_status inch_residual_norms(int64_t len, int32_t k, const double *X, int64_t ldx, const double *Y, int64_t ldy,
+                                const double *theta, double *out, double *work, void *stream);
+
+/* One fused CGS2 re-orthogonalisation of a Lanczos step against the basis V[:J]
+ * (J <= 256), given h1 = V[:J] u from inch_multidot:
+ *   u' = u - V^T h1;  beta = ||u'||;  h2 = V[:J] u';  u = u' - V^T h2;  vnext = u / beta
+ * (vnext may be NULL). beta_out: device scalar. work: inch_cgs2_work_size(J) doubles.
+ * Reads V twice (three times with the first projection) instead of four. */
+int64_t inch_cgs2_work_size(int32_t J);
+inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld, const double *h1, double *u,
+                             double *vnext, double *beta_out, double *work, void *stream);
+
 /* out[q*ld_out + i] = sum_r W[r*k + q] * V[r*ld + i]  for q < k, r < m
  * (W float64 [m, k] row-major on device; out must not alias V). */
 inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, int64_t ld,

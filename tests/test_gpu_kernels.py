@@ -217,3 +217,53 @@ def test_dense_kernels(cuda):
     assert np.allclose(y32.cpu().numpy(), (2.0 * V[1]).astype(np.float32))
     ops.convert(y32, dst)
     assert np.array_equal(dst.cpu().numpy(), y32.cpu().numpy().astype(np.float64))
+
+
+@pytest.mark.parametrize("k1,k2,n", [(64, 64, 100003), (7, 70, 5000), (130, 6, 33333), (1, 1, 17)])
+def test_gram_mfma_and_residual_norms(cuda, k1, k2, n):
+    from inching_amd import ops
+    print("case: gram k1=%d k2=%d n=%d" % (k1, k2, n), flush=True)
+    rng = np.random.default_rng(k1 * 1000 + k2)
+    X = rng.standard_normal((k1, n))
+    Y = rng.standard_normal((k2, n))
+    G = ops.gram(_dev(X), _dev(Y)).cpu().numpy()
+    ref = X @ Y.T
+    assert np.allclose(G, ref, rtol=1e-11, atol=1e-9 * np.sqrt(n))
+    # strided row views (leading dimension > row length)
+    Xb = _dev(np.concatenate([X, X], axis=1))
+    G2 = ops.gram(Xb[:, :n], Xb[:, :n]).cpu().numpy()
+    assert np.allclose(G2, X @ X.T, rtol=1e-11, atol=1e-9 * np.sqrt(n))
+    th = rng.standard_normal(k1)
+    Yk = rng.standard_normal((k1, n))
+    out = ops.residual_norms(_dev(X), _dev(Yk), _dev(th)).cpu().numpy()
+    assert np.allclose(out, np.linalg.norm(Yk - th[:, None] * X, axis=1), rtol=1e-12)
+
+
+def test_matfree_multi_rhs_chunks(mat):
+    A, Bs = mat
+    for r in (1, 8, 13, 64):
+        X = np.random.default_rng(r).standard_normal((A.shape[0], r))
+        Y = A.matfree(_dev(X)).cpu().numpy()
+        assert np.linalg.norm(Y - Bs @ X) <= 1e-13 * np.linalg.norm(Bs @ X)
+
+
+@pytest.mark.parametrize("J,n", [(1, 1000), (37, 100003), (128, 90001), (256, 20000)])
+def test_cgs2_reorth_fused(cuda, J, n):
+    from inching_amd import ops
+    print("case: cgs2 J=%d n=%d" % (J, n), flush=True)
+    rng = np.random.default_rng(J)
+    V = np.linalg.qr(rng.standard_normal((n, J)))[0].T.copy()      # orthonormal rows
+    u = rng.standard_normal(n) + 3.0 * V[0]
+    Vd, ud = _dev(np.concatenate([V, np.zeros((1, n))])), _dev(u)
+    h = torch.empty(J, dtype=torch.float64, device="cuda")
+    ws = ops.Workspace(J, Vd.device)
+    ops.multidot(Vd, J, ud, h, ws)
+    beta = torch.empty(1, dtype=torch.float64, device="cuda")
+    ops.cgs2_reorth(Vd, J, h, ud, beta, vnext=Vd[J])
+    u1 = u - V.T @ (V @ u)
+    b_ref = np.linalg.norm(u1)
+    u2 = u1 - V.T @ (V @ u1)
+    assert np.isclose(beta.item(), b_ref, rtol=1e-12)
+    assert np.allclose(ud.cpu().numpy(), u2, rtol=1e-10, atol=1e-12)
+    assert np.allclose(Vd[J].cpu().numpy(), u2 / b_ref, rtol=1e-10, atol=1e-12)
+    assert np.max(np.abs(V @ Vd[J].cpu().numpy())) < 1e-13
