@@ -30,7 +30,7 @@ SIGNATURES = {
     "inch_bsr_spmm": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _vp]),
     "inch_anm_matfree_apply": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _vp, _vp, _i32, _vp]),
     "inch_anm_matfree_spmv": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp,
-                                        _dbl, _vp, _vp, _i32, _vp]),
+                                        _dbl, _vp, _vp, _i32, _i32, _vp]),
     "inch_stream_copy": (_STATUS, [_i64, _vp, _vp, _vp]),
     "inch_multidot_work_size": (_i64, [_i32]),
     "inch_multidot": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
@@ -44,6 +44,13 @@ SIGNATURES = {
     "inch_residual_norms": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp, _vp]),
     "inch_cgs2_work_size": (_i64, [_i32]),
     "inch_cgs2_reorth": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_seg_multidot": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "inch_seg_update": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, _dbl, _vp, _vp, _vp,
+                                  _vp]),
+    "inch_seg_scale": (_STATUS, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_seg_combine": (_STATUS, [_i32, _vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "inch_seg_resnorm": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
+                                   _vp]),
     "inch_rr_combine": (_STATUS, [_i64, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
 }
 

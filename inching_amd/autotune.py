@@ -1,0 +1,100 @@
+"""Launch-shape autotuner for the operator kernels.
+
+The block SpMV takes ``lanes`` (lanes of a 64-wide wavefront per block row) and the
+matrix-free ANM product takes ``lanes*10 + unroll``. The best shape depends on the mean
+neighbour count (row length), the atom count (grid size against 256 CUs) and the value
+precision, so each candidate is timed with HIP events on the matrix at hand (a few
+launches each) and the winner is cached per (kernel, precision, log2 atoms, row length).
+The cache lives in the process; ``INCHING_TUNE_CACHE`` names an optional JSON file that
+persists it.
+"""
+import json
+import math
+import os
+
+import torch
+
+from . import ops
+
+BSR_LANES = (4, 8, 16, 32, 64)
+MATFREE_CONFIGS = (41, 42, 81, 82, 161, 162, 321, 322)
+
+_cache = {}
+_loaded = False
+
+
+def _path():
+    return os.environ.get("INCHING_TUNE_CACHE")
+
+
+def _load():
+    global _loaded
+    if _loaded:
+        return
+    _loaded = True
+    p = _path()
+    if p and os.path.exists(p):
+        try:
+            with open(p) as f:
+                _cache.update(json.load(f))
+        except (OSError, ValueError):
+            pass
+
+
+def _save():
+    p = _path()
+    if p:
+        os.makedirs(os.path.dirname(os.path.abspath(p)), exist_ok=True)
+        with open(p, "w") as f:
+            json.dump(_cache, f, indent=1, sort_keys=True)
+
+
+def key(kind, precision, A):
+    deg = A.nnzb / max(1, A.n_atoms)
+    return "%s/%s/n2^%d/deg%d" % (kind, precision, int(round(math.log2(max(2, A.n_atoms)))), int(round(deg / 8.0)) * 8)
+
+
+def _time(fn, reps):
+    for _ in range(2):
+        fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def best_config(A, precision, reps=5):
+    """Lanes (block SpMV) or lanes*10+unroll (matrix-free) for this matrix and precision."""
+    _load()
+    kind = "matfree" if precision == "mf64" else "bsr"
+    k = key(kind, precision, A)
+    if k in _cache:
+        return int(_cache[k]["best"])
+    n = A.shape[0]
+    vdt = torch.float32 if precision == "f32" else torch.float64
+    x = torch.randn(n, dtype=vdt, device=A.device)
+    z = torch.randn_like(x)
+    y = torch.empty_like(x)
+    acc = torch.zeros_like(x)
+    times = {}
+    if kind == "matfree":
+        for cfg in MATFREE_CONFIGS:
+            times[cfg] = _time(lambda: ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, x, y, s1=0.1, c0=1.0,
+                                                        z=z, c2=1.0, acc=acc, mu=0.1, config=cfg), reps)
+    else:
+        vals = A.vals(precision)
+        for lanes in BSR_LANES:
+            times[lanes] = _time(lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, s1=0.1, c0=1.0, z=z,
+                                                      c2=1.0, acc=acc, mu=0.1, lanes=lanes), reps)
+    best = min(times, key=times.get)
+    _cache[k] = {"best": int(best), "ms": {str(c): round(t, 5) for c, t in times.items()}}
+    _save()
+    return int(best)
+
+
+def table():
+    _load()
+    return dict(_cache)

@@ -1,0 +1,287 @@
+"""Batched solve of many independent structures in one launch sequence.
+
+The reference solves one structure per iteration of its driver loop
+(``Command8A/commandBenchmark_InchingTRLMHD_Linux_Pdb_0064_A100.py:189``). Here all
+structures of a batch form one block-diagonal ANM matrix: their coordinates are placed
+far apart in one frame, so a single cell-list search, RCM ordering (the BFS keeps every
+connected structure contiguous) and assembly build all of them. Thick-restart Lanczos
+then runs every structure in lockstep: the operator is one launch over the whole batch,
+and every inner product, norm, update and restart combination is a *segmented* kernel
+over the structures' dof ranges (``inch_seg_*``). The restart body of all structures is
+captured once into a HIP graph and replayed. The projected eigenproblems (one arrowhead
+matrix per structure) are solved on the host as one batched ``numpy.linalg.eigh``.
+
+Each structure stops contributing to the result when its Lanczos residual estimate
+meets the tolerance (its latency is recorded then); the batch runs until all have.
+The rigid-body block of every structure is deflated with a Hotelling shift, as in the
+reference's TRLMHD.
+"""
+import time
+
+import numpy as np
+import torch
+
+from . import ops
+from ._lib import call
+from .hessian import build_hessian
+from .operator import Operator
+
+
+class Segments:
+    """Per-structure dof ranges cut into chunks of at most ``chunk`` dofs."""
+
+    def __init__(self, seg_ptr, device, chunk=4096):
+        seg_ptr = np.asarray(seg_ptr, dtype=np.int64)
+        self.S = seg_ptr.shape[0] - 1
+        cb, ce, cs, sp = [], [], [], [0]
+        for s in range(self.S):
+            b, e = int(seg_ptr[s]), int(seg_ptr[s + 1])
+            for x in range(b, e, chunk):
+                cb.append(x)
+                ce.append(min(e, x + chunk))
+                cs.append(s)
+            sp.append(len(cb))
+        self.nchunk = len(cb)
+        self.seg_ptr = seg_ptr
+        self.cbeg = torch.as_tensor(np.asarray(cb, dtype=np.int64), device=device)
+        self.cend = torch.as_tensor(np.asarray(ce, dtype=np.int64), device=device)
+        self.cseg = torch.as_tensor(np.asarray(cs, dtype=np.int32), device=device)
+        self.sptr = torch.as_tensor(np.asarray(sp, dtype=np.int64), device=device)
+        self.sizes = np.diff(seg_ptr)
+
+    def work(self, J, device):
+        return torch.empty(self.nchunk * max(1, J), dtype=torch.float64, device=device)
+
+
+P = ops.P
+SP = ops.stream_ptr
+
+
+class SegOps:
+    def __init__(self, seg, maxJ, device):
+        self.seg = seg
+        self.wk = seg.work(maxJ, device)
+
+    def multidot(self, V, J, u, h):
+        s = self.seg
+        call("inch_seg_multidot", s.nchunk, s.S, P(s.cbeg), P(s.cend), P(s.sptr), int(J), P(V), V.stride(0), P(u),
+             P(h), P(self.wk), SP())
+
+    def update(self, V, J, h, hld, scale, u, nrm=None):
+        s = self.seg
+        call("inch_seg_update", s.nchunk, s.S, P(s.cbeg), P(s.cend), P(s.cseg), P(s.sptr), int(J),
+             P(V) if J > 0 else None, V.stride(0) if J > 0 else u.numel(), P(h) if J > 0 else None, int(hld),
+             float(scale), P(u), P(nrm), P(self.wk) if nrm is not None else None, SP())
+
+    def scale(self, src, div, dst, dst2=None):
+        s = self.seg
+        call("inch_seg_scale", s.nchunk, P(s.cbeg), P(s.cend), P(s.cseg), P(src), P(div), P(dst), P(dst2), SP())
+
+    def combine(self, V, m, W, out):
+        s = self.seg
+        k = W.shape[2]
+        call("inch_seg_combine", s.nchunk, P(s.cbeg), P(s.cend), P(s.cseg), int(m), int(k), P(V), V.stride(0),
+             P(W), P(out), out.stride(0), SP())
+
+    def resnorm(self, X, Y, theta, out):
+        s = self.seg
+        k = X.shape[0]
+        call("inch_seg_resnorm", s.nchunk, s.S, P(s.cbeg), P(s.cend), P(s.cseg), P(s.sptr), int(k), P(X),
+             X.stride(0), P(Y), Y.stride(0), P(theta), P(out), P(self.wk), SP())
+
+
+def place_apart(structures, gap=30.0):
+    """Translate each structure (centred) onto a cubic grid with cells wider than every
+    structure plus ``gap`` A, so no pair of atoms from different structures is within the
+    cutoff. Returns (xyz [sum N, 3], atom offsets [S+1])."""
+    S = len(structures)
+    span = max(float(np.max(np.ptp(x, axis=0))) for x in structures) + gap
+    g = int(np.ceil(S ** (1.0 / 3.0)))
+    parts, off = [], [0]
+    for s, x in enumerate(structures):
+        c = np.array([s % g, (s // g) % g, s // (g * g)], dtype=np.float64) * span
+        parts.append(x - x.mean(0) + c)
+        off.append(off[-1] + x.shape[0])
+    return np.concatenate(parts), np.asarray(off, dtype=np.int64)
+
+
+def segmented_rigid_basis(xyz, seg_atoms, segops):
+    """6 rigid-body rows per structure, nonzero only on its own dofs, orthonormalised per
+    structure (two Cholesky-QR sweeps with segmented Gram products)."""
+    n = xyz.shape[0]
+    dev = xyz.device
+    S = len(seg_atoms) - 1
+    comp = torch.as_tensor(np.repeat(np.arange(S), np.diff(seg_atoms)), device=dev)
+    cnt = torch.as_tensor(np.diff(seg_atoms).astype(np.float64), device=dev)
+    Q = torch.zeros((6, 3 * n), dtype=torch.float64, device=dev)
+    inv = (1.0 / torch.sqrt(cnt))[comp]
+    for a in range(3):
+        Q[a].view(n, 3)[:, a] = inv
+    x = xyz
+    zero = torch.zeros_like(x[:, 0])
+    Q[3] = torch.stack([zero, -x[:, 2], x[:, 1]], 1).reshape(-1)
+    Q[4] = torch.stack([x[:, 2], zero, -x[:, 0]], 1).reshape(-1)
+    Q[5] = torch.stack([-x[:, 1], x[:, 0], zero], 1).reshape(-1)
+    G = torch.empty((6, S * 6), dtype=torch.float64, device=dev)
+    for _ in range(2):
+        for a in range(6):
+            segops.multidot(Q, 6, Q[a].contiguous(), G[a])
+        Gh = G.cpu().numpy().reshape(6, S, 6).transpose(1, 0, 2)    # [S, 6, 6]
+        Gh = 0.5 * (Gh + Gh.transpose(0, 2, 1))
+        R = np.linalg.cholesky(Gh).transpose(0, 2, 1)                 # G = R^T R
+        W = torch.as_tensor(np.ascontiguousarray(np.linalg.inv(R)), device=dev)   # [S, 6, 6]
+        Qn = torch.empty_like(Q)
+        segops.combine(Q, 6, W, Qn)
+        Q = Qn
+    return Q
+
+
+class BatchResult:
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
+                factor=2, seed=0, use_graph=True, verbose=False):
+    """Lowest k non-trivial modes of every structure (list of [N_s, 3] coordinate arrays, A).
+
+    Returns BatchResult with per-structure eigenvalues [S, k] (PlusI included), eigenvectors
+    (rows [k, dof] in the batch's RCM order), relative residuals [S, k], per-structure
+    latency (seconds from start until its convergence) and the total time."""
+    ops.require_gpu()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    torch.cuda.synchronize()
+    t0 = time.time()
+    X, off = place_apart(structures)
+    prec = {"mf64": (), "f32": ("f32",), "f64": ("f64",)}[precision]
+    A = build_hessian(X, rc, plus_i=plus_i, reorder=True, precision=prec, device=dev)
+    # structures stay contiguous after RCM; locate each by its first atom's new position
+    src_of = np.repeat(np.arange(len(structures)), np.diff(off))[A.order]       # structure of each new row
+    starts = np.nonzero(np.r_[True, src_of[1:] != src_of[:-1]])[0]
+    seg_struct = src_of[starts]
+    seg_atoms = np.r_[starts, A.n_atoms].astype(np.int64)
+    if len(starts) != len(structures):
+        raise RuntimeError("structures are not contiguous after ordering (are they closer than the cutoff?)")
+    S = len(structures)
+    n = A.shape[0]
+    seg = Segments(3 * seg_atoms, dev)
+    min_dof = int(seg.sizes.min())
+    ncv = min(max(factor * k, k + 32), min_dof - 7)
+    if ncv <= k + 1:
+        raise AssertionError("ABORTED. k must be smaller than every structure's dof count minus the rigid modes")
+    so = SegOps(seg, max(ncv, 6) + 1, dev)
+    Q = segmented_rigid_basis(A.xyz, seg_atoms, so)
+    op = Operator(A, precision)
+    f64 = dict(dtype=torch.float64, device=dev)
+    V = torch.empty((ncv, n), **f64)
+    Vk = torch.empty((k, n), **f64)
+    u = torch.empty(n, **f64)
+    h = torch.empty(S * (ncv + 1), **f64)
+    hq = torch.empty(S * 6, **f64)
+    alpha = torch.zeros((ncv, S), **f64)
+    beta = torch.zeros((ncv, S), **f64)
+    scal = torch.empty(S, **f64)
+    betak_d = torch.zeros((S, k), **f64)
+
+    def apply(x, y):
+        op.apply(x, y)
+        if shift:
+            so.multidot(Q, 6, x, hq)
+            so.update(Q, 6, hq, 6, -shift, y)
+
+    def lanczos(i0, i1):
+        for i in range(i0, i1):
+            apply(V[i], u)
+            J = i + 1
+            so.multidot(V, J, u, h)
+            alpha[i].copy_(h[:S * J].view(S, J)[:, i])
+            so.update(V, J, h, J, 1.0, u, nrm=beta[i])
+            so.multidot(V, J, u, h)
+            so.update(V, J, h, J, 1.0, u)
+            if i < i1 - 1:
+                so.scale(u, beta[i], V[i + 1])
+
+    def body():
+        so.multidot(V, k, u, h)
+        so.update(V, k, h, k, 1.0, u, nrm=scal)
+        so.scale(u, scal, V[k])
+        apply(V[k], u)
+        so.multidot(V[k:k + 1], 1, u, alpha[k])
+        so.update(V[k:k + 1], 1, alpha[k], 1, 1.0, u)
+        so.update(V, k, betak_d, k, 1.0, u, nrm=beta[k])
+        so.scale(u, beta[k], V[k + 1])
+        lanczos(k + 1, ncv)
+
+    def restart(a, b, bk):
+        T = np.zeros((S, ncv, ncv))
+        idx = np.arange(ncv)
+        T[:, idx, idx] = a.T
+        T[:, idx[:-1], idx[1:]] = b[:-1].T
+        T[:, idx[1:], idx[:-1]] = b[:-1].T
+        if bk is not None:
+            T[:, k, :k] = bk
+            T[:, :k, k] = bk
+        w, Z = np.linalg.eigh(T)                 # ascending per structure
+        Sel, W = w[:, :k], Z[:, :, :k]
+        Wd = torch.as_tensor(np.ascontiguousarray(W), device=dev)
+        so.combine(V, ncv, Wd, Vk)
+        V[:k].copy_(Vk)
+        bkn = b[-1][:, None] * W[:, -1, :]       # [S, k]
+        return Sel, bkn
+
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    u.copy_(torch.rand(n, generator=g, dtype=torch.float64, device=dev))
+    so.update(V, 0, None, 1, 0.0, u, nrm=scal)
+    so.scale(u, scal, V[0])
+    lanczos(0, ncv)
+    a, b = alpha.cpu().numpy(), beta.cpu().numpy()
+    Sel, bk = restart(a, b, None)
+    lam_max = None
+    done_at = np.full(S, np.nan)
+    res = np.linalg.norm(bk, axis=1)
+    graph = None
+    it = 0
+    # lam_max for the relative residual: a 300-step Lanczos bound of the whole batch
+    from .chebyshev import spectrum_bound
+    lam_max = spectrum_bound(A, precision=precision)[1]
+    tol_abs = tol_rel * lam_max
+    conv = res <= tol_abs
+    done_at[conv] = time.time() - t0
+    for it in range(maxiter):
+        if conv.all():
+            break
+        betak_d.copy_(torch.as_tensor(bk, device=dev))
+        if use_graph:
+            if graph is None:
+                graph = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(graph):
+                    body()
+            graph.replay()
+        else:
+            body()
+        a, b = alpha.cpu().numpy().copy(), beta.cpu().numpy().copy()
+        a[:k] = Sel.T
+        b[:k] = 0.0
+        Sel, bk = restart(a, b, bk)
+        res = np.linalg.norm(bk, axis=1)
+        newly = (~conv) & (res <= tol_abs)
+        done_at[newly] = time.time() - t0
+        conv |= newly
+        if verbose:
+            print("batch restart %d: %d/%d converged, max est %.2e" % (it, conv.sum(), S, res.max()))
+    # float64 verification of every pair (matrix-free operator, segmented residual norms)
+    Xr = V[:k]
+    Yc = A.matfree(Xr.T.contiguous())
+    Y = Yc.T.contiguous()
+    theta = torch.as_tensor(np.ascontiguousarray(Sel), device=dev)
+    rn = torch.empty(S * k, **f64)
+    so.resnorm(Xr, Y, theta, rn)
+    torch.cuda.synchronize()
+    total = time.time() - t0
+    rel = rn.cpu().numpy().reshape(S, k) / lam_max
+    order = np.argsort(seg_struct)
+    done = np.where(np.isnan(done_at), total, done_at)
+    return BatchResult(eigval=Sel[order], rel_res=rel[order], latency=done[order], seconds=total,
+                       restarts=it + 1, n_spmv=op.n_spmv, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
+                       seg_struct=seg_struct, A=A, eigvec=Xr, converged=int(conv.sum()))

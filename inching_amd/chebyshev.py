@@ -342,11 +342,14 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
     vals = A.vals(precision) if precision != "mf64" else None
     row = v.view(1, -1)
 
+    from .autotune import best_config
+    cfg = best_config(A, precision)
+
     def av(x, y):
         if vals is None:
-            ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, x, y)
+            ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, x, y, config=cfg)
         else:
-            ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=A.lanes)
+            ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=cfg)
 
     av(v, f)
     ops.multidot(row, 1, f, alpha[0:1], ws)

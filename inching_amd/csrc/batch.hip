@@ -1,0 +1,228 @@
+// Segmented (batched) FP64 kernels: many independent structures stored as contiguous dof
+// segments of one long vector. Work is cut into chunks that never straddle a segment;
+// chunk c covers dofs [cbeg[c], cend[c]) of segment cseg[c], and the chunks of segment s
+// are sptr[s] .. sptr[s+1]-1. Every reduction is two-phase with fixed orders
+// (deterministic): per-chunk partials, then a per-segment sum over its chunks.
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) INCH_FAIL(INCH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+namespace {
+
+constexpr int kSegMaxJ = 512;
+
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// part[c*J + j] = sum_{i in chunk c} V[j*ld + i] * u[i]
+__global__ void __launch_bounds__(256) k_seg_dot_part(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
+                                                       int J, const double *__restrict__ V, int64_t ld,
+                                                       const double *__restrict__ u, double *__restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int64_t b = cbeg[c], e = cend[c];
+  for (int j0 = 0; j0 < J; j0 += 8) {
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+      const double ui = u[i];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (j0 + t < J) a[t] += V[(int64_t)(j0 + t) * ld + i] * ui;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const double s = block_sum(a[t], red);
+      if (threadIdx.x == 0 && j0 + t < J) part[(int64_t)c * J + j0 + t] = s;
+    }
+  }
+}
+
+// h[s*J + j] = sum over the chunks of segment s (optionally sqrt)
+__global__ void k_seg_reduce(int S, int J, const int64_t *__restrict__ sptr, const double *__restrict__ part,
+                             double *__restrict__ h, int do_sqrt) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= (int64_t)S * J) return;
+  const int s = (int)(t / J), j = (int)(t % J);
+  double acc = 0.0;
+  for (int64_t c = sptr[s]; c < sptr[s + 1]; ++c) acc += part[c * J + j];
+  h[t] = do_sqrt ? sqrt(acc) : acc;
+}
+
+// u[i] -= scale * sum_j V[j*ld + i] * h[s*hld + j] for i in chunk c of segment s; part[c] = sum u_new^2
+__global__ void __launch_bounds__(256) k_seg_update(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
+                                                     const int32_t *__restrict__ cseg, int J,
+                                                     const double *__restrict__ V, int64_t ld,
+                                                     const double *__restrict__ h, int hld, double scale,
+                                                     double *__restrict__ u, double *__restrict__ part) {
+  __shared__ double hs[kSegMaxJ];
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  for (int t = threadIdx.x; t < J; t += blockDim.x) hs[t] = h[(int64_t)s * hld + t] * scale;
+  __syncthreads();
+  double ss = 0.0;
+  for (int64_t i = cbeg[c] + threadIdx.x; i < cend[c]; i += blockDim.x) {
+    double acc = 0.0;
+    for (int j = 0; j < J; ++j) acc += V[(int64_t)j * ld + i] * hs[j];
+    const double un = u[i] - acc;
+    u[i] = un;
+    ss += un * un;
+  }
+  if (part) {
+    const double tot = block_sum(ss, red);
+    if (threadIdx.x == 0) part[c] = tot;
+  }
+}
+
+// dst[i] = src[i] / div[s] (dst2 too when given)
+__global__ void __launch_bounds__(256) k_seg_scale(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
+                                                    const int32_t *__restrict__ cseg, const double *__restrict__ src,
+                                                    const double *__restrict__ div, double *__restrict__ dst,
+                                                    double *__restrict__ dst2) {
+  const int c = blockIdx.x;
+  const double inv = 1.0 / div[cseg[c]];
+  for (int64_t i = cbeg[c] + threadIdx.x; i < cend[c]; i += blockDim.x) {
+    const double v = src[i] * inv;
+    dst[i] = v;
+    if (dst2) dst2[i] = v;
+  }
+}
+
+// out[q*ld_out + i] = sum_r W[s][r][q] * V[r*ld + i]  (W: [S, m, k] row-major), q < k <= KM
+template <int KM>
+__global__ void __launch_bounds__(256) k_seg_combine(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
+                                                      const int32_t *__restrict__ cseg, int m, int k,
+                                                      const double *__restrict__ V, int64_t ld,
+                                                      const double *__restrict__ W, double *__restrict__ out,
+                                                      int64_t ld_out) {
+  extern __shared__ double ws[];  // [m][k]
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  for (int t = threadIdx.x; t < m * k; t += blockDim.x) ws[t] = W[(int64_t)s * m * k + t];
+  __syncthreads();
+  for (int64_t i = cbeg[c] + threadIdx.x; i < cend[c]; i += blockDim.x) {
+    double a[KM];
+#pragma unroll
+    for (int q = 0; q < KM; ++q) a[q] = 0.0;
+    for (int r = 0; r < m; ++r) {
+      const double v = V[(int64_t)r * ld + i];
+#pragma unroll
+      for (int q = 0; q < KM; ++q)
+        if (q < k) a[q] += ws[r * k + q] * v;
+    }
+#pragma unroll
+    for (int q = 0; q < KM; ++q)
+      if (q < k) out[(int64_t)q * ld_out + i] = a[q];
+  }
+}
+
+// part[c*k + q] = sum_{i in chunk} (Y[q,i] - theta[s*k+q] X[q,i])^2
+__global__ void __launch_bounds__(256) k_seg_resnorm(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
+                                                      const int32_t *__restrict__ cseg, int k,
+                                                      const double *__restrict__ X, int64_t ldx,
+                                                      const double *__restrict__ Y, int64_t ldy,
+                                                      const double *__restrict__ theta, double *__restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  for (int q = 0; q < k; ++q) {
+    const double th = theta[(int64_t)s * k + q];
+    double acc = 0.0;
+    for (int64_t i = cbeg[c] + threadIdx.x; i < cend[c]; i += blockDim.x) {
+      const double r = Y[(int64_t)q * ldy + i] - th * X[(int64_t)q * ldx + i];
+      acc += r * r;
+    }
+    const double tot = block_sum(acc, red);
+    if (threadIdx.x == 0) part[(int64_t)c * k + q] = tot;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+inch_status inch_seg_multidot(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend,
+                              const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *u, double *h,
+                              double *work, void *stream) {
+  INCH_REQUIRE(nchunk > 0 && S > 0 && J > 0 && cbeg && cend && sptr && V && u && h && work,
+               "inch_seg_multidot: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  k_seg_dot_part<<<nchunk, 256, 0, st>>>(cbeg, cend, J, V, ld, u, work);
+  HIP_TRY(hipGetLastError());
+  const int64_t tot = (int64_t)S * J;
+  k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, J, sptr, work, h, 0);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                            const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
+                            double scale, double *u, double *nrm_out, double *work, void *stream) {
+  INCH_REQUIRE(nchunk > 0 && S > 0 && J >= 0 && J <= kSegMaxJ && cbeg && cend && cseg && u,
+               "inch_seg_update: bad args (J <= 512)");
+  INCH_REQUIRE(J == 0 || (V && h && hld >= J), "inch_seg_update: V/h missing");
+  INCH_REQUIRE(!nrm_out || (work && sptr), "inch_seg_update: norm needs work and sptr");
+  hipStream_t st = (hipStream_t)stream;
+  k_seg_update<<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, J, V, ld, h, hld, scale, u, nrm_out ? work : nullptr);
+  HIP_TRY(hipGetLastError());
+  if (nrm_out) {
+    k_seg_reduce<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, 1, sptr, work, nrm_out, 1);
+    HIP_TRY(hipGetLastError());
+  }
+  return INCH_OK;
+}
+
+inch_status inch_seg_scale(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                           const double *src, const double *div, double *dst, double *dst2, void *stream) {
+  INCH_REQUIRE(nchunk > 0 && cbeg && cend && cseg && src && div && dst, "inch_seg_scale: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  k_seg_scale<<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, src, div, dst, dst2);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_seg_combine(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg, int32_t m,
+                             int32_t k, const double *V, int64_t ld, const double *W, double *out, int64_t ld_out,
+                             void *stream) {
+  INCH_REQUIRE(nchunk > 0 && cbeg && cend && cseg && V && W && out && m > 0 && k > 0 && k <= 64,
+               "inch_seg_combine: bad args (k <= 64)");
+  INCH_REQUIRE((int64_t)m * k * 8 <= 64 * 1024, "inch_seg_combine: m*k too large for LDS");
+  hipStream_t st = (hipStream_t)stream;
+  if (k <= 8)
+    k_seg_combine<8><<<nchunk, 256, (size_t)m * k * sizeof(double), st>>>(cbeg, cend, cseg, m, k, V, ld, W, out, ld_out);
+  else if (k <= 32)
+    k_seg_combine<32><<<nchunk, 256, (size_t)m * k * sizeof(double), st>>>(cbeg, cend, cseg, m, k, V, ld, W, out,
+                                                                           ld_out);
+  else
+    k_seg_combine<64><<<nchunk, 256, (size_t)m * k * sizeof(double), st>>>(cbeg, cend, cseg, m, k, V, ld, W, out,
+                                                                           ld_out);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_seg_resnorm(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                             const int64_t *sptr, int32_t k, const double *X, int64_t ldx, const double *Y,
+                             int64_t ldy, const double *theta, double *out, double *work, void *stream) {
+  INCH_REQUIRE(nchunk > 0 && S > 0 && k > 0 && cbeg && cend && cseg && sptr && X && Y && theta && out && work,
+               "inch_seg_resnorm: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  k_seg_resnorm<<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, k, X, ldx, Y, ldy, theta, work);
+  HIP_TRY(hipGetLastError());
+  const int64_t tot = (int64_t)S * k;
+  k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, k, sptr, work, out, 1);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+}  // extern "C"

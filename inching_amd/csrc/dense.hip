@@ -174,35 +174,40 @@ inch_status inch_residual_norms(int64_t len, int32_t k, const double *X, int64_t
 namespace {
 
 constexpr int kCgsMaxJ = 256;
-constexpr int kCgsCols = 16;
-constexpr int kCgsBlocks = 256;
+constexpr int kCgsBlocks = 2048;
 
+// Pass B over column tiles of C columns (C * J <= 4096 doubles of LDS): stage V[:J, tile]
+// in LDS, form u' for the tile (256/C lanes per column, fixed butterfly), then thread j
+// accumulates V[j, tile] . u'[tile] for its row. Partials per workgroup are summed in a
+// fixed order by k_sum_parts_wg.
+template <int C>
 __global__ void __launch_bounds__(256) k_cgs_pass_b(int64_t len, int J, const double *__restrict__ V, int64_t ld,
                                                      const double *__restrict__ h1, double *__restrict__ u,
                                                      double *__restrict__ part) {
-  __shared__ double vs[kCgsMaxJ][kCgsCols + 1];
+  constexpr int LPC = 256 / C;  // lanes per column
+  __shared__ double vs[4096 / C][C + 1];
   __shared__ double hs[kCgsMaxJ];
-  __shared__ double up[kCgsCols];
+  __shared__ double up[C];
   const int tid = threadIdx.x;
   for (int t = tid; t < J; t += 256) hs[t] = h1[t];
   double acc = 0.0, ss = 0.0;
-  const int64_t ntiles = (len + kCgsCols - 1) / kCgsCols;
+  const int64_t ntiles = (len + C - 1) / C;
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int64_t c0 = tile * kCgsCols;
+    const int64_t c0 = tile * C;
     __syncthreads();
     {
-      const int c = tid % kCgsCols;
+      const int c = tid % C;
       const int64_t i = c0 + c;
-      for (int r = tid / kCgsCols; r < J; r += 256 / kCgsCols) vs[r][c] = (i < len) ? V[(int64_t)r * ld + i] : 0.0;
+      for (int r = tid / C; r < J; r += 256 / C) vs[r][c] = (i < len) ? V[(int64_t)r * ld + i] : 0.0;
     }
     __syncthreads();
-    // u'[c] = u[c] - sum_j vs[j][c] h1[j]: 16 lanes per column, fixed butterfly
     {
-      const int c = tid / 16, part_lane = tid % 16;
+      const int c = tid / LPC, pl = tid % LPC;
       double s = 0.0;
-      for (int r = part_lane; r < J; r += 16) s += vs[r][c] * hs[r];
-      for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
-      if (part_lane == 0) {
+      for (int r = pl; r < J; r += LPC) s += vs[r][c] * hs[r];
+#pragma unroll
+      for (int off = LPC / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPC);
+      if (pl == 0) {
         const int64_t i = c0 + c;
         const double un = (i < len) ? u[i] - s : 0.0;
         up[c] = un;
@@ -212,11 +217,11 @@ __global__ void __launch_bounds__(256) k_cgs_pass_b(int64_t len, int J, const do
     __syncthreads();
     if (tid < J) {
 #pragma unroll
-      for (int c = 0; c < kCgsCols; ++c) acc += vs[tid][c] * up[c];
+      for (int c = 0; c < C; ++c) acc += vs[tid][c] * up[c];
     }
     if (tid == 0) {
 #pragma unroll
-      for (int c = 0; c < kCgsCols; ++c) ss += up[c] * up[c];
+      for (int c = 0; c < C; ++c) ss += up[c] * up[c];
     }
   }
   double *dst = part + (int64_t)blockIdx.x * (J + 1);
@@ -253,11 +258,17 @@ inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld
   INCH_REQUIRE(len > 0 && J > 0 && J <= kCgsMaxJ && V && h1 && u && beta_out && work && ld >= len,
                "inch_cgs2_reorth: bad args (J must be 1..256)");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t ntiles = (len + kCgsCols - 1) / kCgsCols;
+  const int C = J <= 64 ? 64 : (J <= 128 ? 32 : 16);
+  const int64_t ntiles = (len + C - 1) / C;
   const int gx = (int)(ntiles < kCgsBlocks ? ntiles : kCgsBlocks);
   double *part = work;
   double *h2 = work + (int64_t)kCgsBlocks * (J + 1);
-  k_cgs_pass_b<<<gx, 256, 0, st>>>(len, J, V, ld, h1, u, part);
+  if (C == 64)
+    k_cgs_pass_b<64><<<gx, 256, 0, st>>>(len, J, V, ld, h1, u, part);
+  else if (C == 32)
+    k_cgs_pass_b<32><<<gx, 256, 0, st>>>(len, J, V, ld, h1, u, part);
+  else
+    k_cgs_pass_b<16><<<gx, 256, 0, st>>>(len, J, V, ld, h1, u, part);
   HIP_TRY(hipGetLastError());
   k_sum_parts_wg<<<(unsigned)(J + 1), 256, 0, st>>>(gx, (int64_t)(J + 1), part, h2, 0);
   HIP_TRY(hipGetLastError());

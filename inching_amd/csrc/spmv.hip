@@ -197,7 +197,9 @@ __global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__rest
 
 // Single right-hand side with the same fused epilogue as k_bsr_spmv:
 //   out = s1 * (A x - c0 * x) - c2 * z (+ Q term); y = out; acc += mu * out.
-template <typename VT, int G>
+// G lanes per row; each lane handles U neighbours per iteration with independent loads
+// (memory-level parallelism for the dependent index -> coordinate/vector gathers).
+template <typename VT, int G, int U>
 __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__restrict__ xyz,
                                                       const int64_t *__restrict__ rowptr,
                                                       const int32_t *__restrict__ col, double plus_i,
@@ -211,17 +213,23 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
   const double pi0 = xyz[3 * row], pi1 = xyz[3 * row + 1], pi2 = xyz[3 * row + 2];
   const double vi0 = (double)x[3 * row], vi1 = (double)x[3 * row + 1], vi2 = (double)x[3 * row + 2];
   double a0 = 0, a1 = 0, a2 = 0;
-  for (int64_t b = rowptr[row] + lane; b < rowptr[row + 1]; b += G) {
-    const int64_t j = col[b];
-    if (j == row) continue;
-    const double d0 = xyz[3 * j] - pi0, d1 = xyz[3 * j + 1] - pi1, d2 = xyz[3 * j + 2] - pi2;
-    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-    if (!(r2 > 0.0)) continue;
-    const double t = (d0 * (vi0 - (double)x[3 * j]) + d1 * (vi1 - (double)x[3 * j + 1]) +
-                      d2 * (vi2 - (double)x[3 * j + 2])) * recip_r2(r2);
-    a0 += d0 * t;
-    a1 += d1 * t;
-    a2 += d2 * t;
+  const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
+  for (int64_t b = b0 + lane; b < b1; b += G * U) {
+    int64_t jj[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) jj[q] = (b + q * G < b1) ? (int64_t)col[b + q * G] : row;
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int64_t j = jj[q];
+      const double d0 = xyz[3 * j] - pi0, d1 = xyz[3 * j + 1] - pi1, d2 = xyz[3 * j + 2] - pi2;
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const double w0 = vi0 - (double)x[3 * j], w1 = vi1 - (double)x[3 * j + 1], w2 = vi2 - (double)x[3 * j + 2];
+      // j == row gives d = 0 (no contribution); r2 == 0 is masked as in the reference
+      const double t = r2 > 0.0 ? (d0 * w0 + d1 * w1 + d2 * w2) * recip_r2(r2) : 0.0;
+      a0 += d0 * t;
+      a1 += d1 * t;
+      a2 += d2 * t;
+    }
   }
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) {
@@ -460,6 +468,39 @@ inch_status launch_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, co
   return INCH_OK;
 }
 
+template <typename VT, int G, int U>
+void launch_matfree(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col, double plus_i,
+                           const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc,
+                           double mu, const double *Q, const double *qcoef, int m_q, hipStream_t st) {
+  const int bs = 256;
+  const int64_t grid = (n * G + bs - 1) / bs;
+  k_matfree_epi<VT, G, U><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, (const VT *)x, (VT *)y, s1, c0,
+                                               (const VT *)z, c2, (VT *)acc, mu, Q, qcoef, m_q);
+}
+
+template <typename VT>
+bool dispatch_matfree(int code, int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                             double plus_i, const void *x, void *y, double s1, double c0, const void *z, double c2,
+                             void *acc, double mu, const double *Q, const double *qcoef, int m_q, hipStream_t st) {
+#define INCH_MF(GG, UU)                                                                                           \
+  case GG * 10 + UU:                                                                                              \
+    launch_matfree<VT, GG, UU>(n, xyz, rowptr, col, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, st); \
+    return true;
+  switch (code) {
+    INCH_MF(4, 1)
+    INCH_MF(4, 2)
+    INCH_MF(8, 1)
+    INCH_MF(8, 2)
+    INCH_MF(16, 1)
+    INCH_MF(16, 2)
+    INCH_MF(32, 1)
+    INCH_MF(32, 2)
+    default:
+      return false;
+  }
+#undef INCH_MF
+}
+
 }  // namespace
 
 extern "C" {
@@ -508,21 +549,20 @@ inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *
 inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
                                   double plus_i, int32_t vdt, const void *x, void *y, double s1, double c0,
                                   const void *z, double c2, void *acc, double mu, const double *Q,
-                                  const double *qcoef, int32_t m_q, void *stream) {
+                                  const double *qcoef, int32_t m_q, int32_t config, void *stream) {
   INCH_REQUIRE(n > 0 && xyz && rowptr && col && x && y && x != y, "inch_anm_matfree_spmv: bad args");
   INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_spmv: qcoef needs Q");
   hipStream_t st = (hipStream_t)stream;
-  constexpr int G = 16;
-  const int bs = 256;
-  const int64_t grid = (n * G + bs - 1) / bs;
+  const int code = config > 0 ? config : 161;
+  bool ok;
   if (vdt == INCH_F64)
-    k_matfree_epi<double, G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, (const double *)x, (double *)y, s1,
-                                                  c0, (const double *)z, c2, (double *)acc, mu, Q, qcoef, m_q);
+    ok = dispatch_matfree<double>(code, n, xyz, rowptr, col, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, st);
   else if (vdt == INCH_F32)
-    k_matfree_epi<float, G><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, (const float *)x, (float *)y, s1, c0,
-                                                 (const float *)z, c2, (float *)acc, mu, Q, qcoef, m_q);
+    ok = dispatch_matfree<float>(code, n, xyz, rowptr, col, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, st);
   else
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: bad dtype");
+  if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: config %d is not lanes*10+unroll with lanes in "
+                                       "{4,8,16,32} and unroll in {1,2}", code);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

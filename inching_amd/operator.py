@@ -20,7 +20,7 @@ from . import ops
 
 
 class Operator:
-    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None, timer=None):
+    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None, timer=None, tune=True):
         if precision not in ("f32", "f64", "mf64"):
             raise ValueError("precision must be 'f32', 'f64' or 'mf64'")
         self.A = A
@@ -35,7 +35,13 @@ class Operator:
         self.pol = pol
         self.Q = Q
         self.shift = float(shift)
-        self.lanes = lanes or A.lanes
+        if lanes is not None:
+            self.lanes = int(lanes)
+        elif tune:
+            from .autotune import best_config
+            self.lanes = best_config(A, precision)
+        else:
+            self.lanes = 161 if precision == "mf64" else A.lanes
         self.n_spmv = 0
         self.n_apply = 0
         dev = A.device
@@ -61,7 +67,7 @@ class Operator:
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
         if self.vals is None:
-            ops.matfree_spmv(self.A.xyz, self.A.rowptr, self.A.col, self.A.plus_i, x, y, **epi)
+            ops.matfree_spmv(self.A.xyz, self.A.rowptr, self.A.col, self.A.plus_i, x, y, config=self.lanes, **epi)
         else:
             ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.lanes, **epi)
         if self.timer is not None:

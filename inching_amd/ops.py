@@ -165,7 +165,7 @@ def matfree_apply(xyz, rowptr, col, plus_i, X, Y, stream=None):
 
 
 def matfree_spmv(xyz, rowptr, col, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None,
-                 qcoef=None, stream=None):
+                 qcoef=None, config=0, stream=None):
     """Matrix-free ANM product with the fused epilogue (see bsr_spmv)."""
     n = xyz.shape[0]
     nd = 3 * n
@@ -182,7 +182,7 @@ def matfree_spmv(xyz, rowptr, col, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0,
             raise ValueError("Q must be float64 [m, 3n]")
         mq = Q.shape[0]
     call("inch_anm_matfree_spmv", n, P(xyz), P(rowptr), P(col), float(plus_i), vd, P(x), P(y), float(s1),
-         float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, stream_ptr(stream))
+         float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(config), stream_ptr(stream))
 
 
 def stream_copy(src, dst, stream=None):

@@ -67,7 +67,7 @@ def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
 
 def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, bound=None, wanted_right=None,
                  max_degree=5120, min_degree=5, ratio=0.7, hotelling=None, factor=2, maxiter=200, seed=0,
-                 phase1_rel=3e-7, v0=None, verbose=False, phase2="mf64", timer=None):
+                 phase1_rel=3e-7, v0=None, verbose=False, phase2="mf64", timer=None, kpm_count=None):
     """k lowest non-trivial eigenpairs of A = H + PlusI I (RCM order) on the device.
 
     Returns ModeResult with eigval (ascending, includes PlusI), eigvec rows [k, n] (float64
@@ -90,7 +90,8 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
             # KPM bisection for an interval holding about 3 (k + rigid) eigenvalues, as in the
             # reference's CTRLM set-up (WantedNumber = (k - 5) * 3, margin 25, 5 probes)
             pol, cnt, interval, nk = CH.optimize_on_memory(
-                A, bound, A.plus_i, wanted_count=max(3 * k, k + 32), designated_start=bound[1] / 10.0,
+                A, bound, A.plus_i, wanted_count=int(kpm_count) if kpm_count else max(3 * k, k + 32),
+                designated_start=bound[1] / 10.0,
                 max_degree=max_degree, min_degree=min_degree, margin=max(5, k // 2), trials=5, ratio=ratio,
                 seed=seed, precision="f32" if precision == "mixed" else precision)
             n_spmv["kpm"] = nk

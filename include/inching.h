@@ -133,11 +133,13 @@ inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *
                                    int32_t nrhs, void *stream);
 
 /* Matrix-free ANM product with the fused epilogue of inch_bsr_spmv (vectors float32
- * or float64 per vdt; geometry and accumulation in float64). */
+ * or float64 per vdt; geometry and accumulation in float64). config = lanes*10 + unroll
+ * (lanes per row 4/8/16/32, neighbours per lane iteration 1/2; 0 = 161), the autotuned
+ * launch shape. */
 inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
                                   double plus_i, int32_t vdt, const void *x, void *y, double s1, double c0,
                                   const void *z, double c2, void *acc, double mu, const double *Q,
-                                  const double *qcoef, int32_t m_q, void *stream);
+                                  const double *qcoef, int32_t m_q, int32_t config, void *stream);
 
 /* dst = src (bytes, multiple of 16): streaming copy for measuring achievable HBM bandwidth. */
 inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *stream);
@@ -190,6 +192,33 @@ inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld
  * (W float64 [m, k] row-major on device; out must not alias V). */
 inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, int64_t ld,
                             const double *W, double *out, int64_t ld_out, void *stream);
+
+/* ---------------------------------------------- batched (segmented) FP64 ---- */
+/* Many structures stored as contiguous dof segments of one vector. Chunk c covers dofs
+ * [cbeg[c], cend[c]) of segment cseg[c]; segment s owns chunks sptr[s] .. sptr[s+1]-1.
+ * Replaces, for a batch of structures at once, the per-structure cuBLAS dot/nrm2/gemv
+ * calls of Burn/ThickRestartLanczosHotellingDeflation/T1.py:465-520 and the restart GEMM
+ * of :176,246 (the reference solves one structure per driver loop iteration). */
+/* h[s*J + j] = sum_{i in s} V[j*ld + i] u[i]; work >= nchunk*J doubles. */
+inch_status inch_seg_multidot(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend,
+                              const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *u, double *h,
+                              double *work, void *stream);
+/* u[i] -= scale * sum_j V[j*ld+i] h[s*hld+j] (J <= 512); nrm_out[s] = ||u_s|| if given
+ * (work >= nchunk doubles). */
+inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                            const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
+                            double scale, double *u, double *nrm_out, double *work, void *stream);
+/* dst[i] = src[i] / div[s] (and dst2). */
+inch_status inch_seg_scale(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                           const double *src, const double *div, double *dst, double *dst2, void *stream);
+/* out[q*ld_out+i] = sum_r W[s][r][q] V[r*ld+i] (W [S, m, k], k <= 64, m*k <= 8192). */
+inch_status inch_seg_combine(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg, int32_t m,
+                             int32_t k, const double *V, int64_t ld, const double *W, double *out, int64_t ld_out,
+                             void *stream);
+/* out[s*k+q] = || Y[q] - theta[s*k+q] X[q] || over segment s; work >= nchunk*k doubles. */
+inch_status inch_seg_resnorm(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                             const int64_t *sptr, int32_t k, const double *X, int64_t ldx, const double *Y,
+                             int64_t ldy, const double *theta, double *out, double *work, void *stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,55 @@
+"""Batched lockstep TRLM over many structures against per-structure dense eigh (oracle)."""
+import numpy as np
+import pytest
+
+from oracle import hessian as OH
+from tests.helpers import small_globule
+
+pytestmark = pytest.mark.gpu
+
+
+def test_batched_solve_matches_dense(cuda):
+    from inching_amd.batch import solve_batch
+    sizes = [420, 700, 300, 1100, 520, 650]
+    structs = [small_globule(n, seed=60 + i) for i, n in enumerate(sizes)]
+    k = 8
+    print("case: batch of %d structures k=%d" % (len(sizes), k), flush=True)
+    r = solve_batch(structs, k=k, tol_rel=1e-10, maxiter=400)
+    assert r.converged == len(sizes)
+    for s, X in enumerate(structs):
+        w = np.linalg.eigvalsh(OH.hessian_dense(X, 8.0, 1.0))
+        assert np.allclose(r.eigval[s], w[6:6 + k], atol=1e-8), (s, r.eigval[s] - w[6:6 + k])
+    assert r.rel_res.max() < 1e-9
+    assert np.all(r.latency > 0) and np.all(r.latency <= r.seconds + 1e-9)
+
+
+def test_batched_segmented_kernels(cuda):
+    import torch
+    from inching_amd.batch import SegOps, Segments
+    rng = np.random.default_rng(3)
+    seg_ptr = np.array([0, 5000, 5003, 16000, 30001])
+    n = seg_ptr[-1]
+    seg = Segments(seg_ptr, torch.device("cuda"), chunk=1024)
+    so = SegOps(seg, 16, torch.device("cuda"))
+    J = 11
+    V = rng.standard_normal((J, n))
+    u = rng.standard_normal(n)
+    Vd, ud = torch.as_tensor(V, device="cuda"), torch.as_tensor(u, device="cuda")
+    h = torch.empty(4 * J, dtype=torch.float64, device="cuda")
+    so.multidot(Vd, J, ud, h)
+    ref = np.stack([V[:, seg_ptr[s]:seg_ptr[s + 1]] @ u[seg_ptr[s]:seg_ptr[s + 1]] for s in range(4)])
+    assert np.allclose(h.cpu().numpy().reshape(4, J), ref, rtol=1e-12, atol=1e-10)
+    nrm = torch.empty(4, dtype=torch.float64, device="cuda")
+    so.update(Vd, J, h, J, 0.5, ud, nrm=nrm)
+    uref = u.copy()
+    for s in range(4):
+        sl = slice(seg_ptr[s], seg_ptr[s + 1])
+        uref[sl] -= 0.5 * V[:, sl].T @ ref[s]
+    assert np.allclose(ud.cpu().numpy(), uref, rtol=1e-11, atol=1e-10)
+    assert np.allclose(nrm.cpu().numpy(), [np.linalg.norm(uref[seg_ptr[s]:seg_ptr[s + 1]]) for s in range(4)])
+    W = rng.standard_normal((4, J, 5))
+    out = torch.empty((5, n), dtype=torch.float64, device="cuda")
+    so.combine(Vd, J, torch.as_tensor(W, device="cuda"), out)
+    for s in range(4):
+        sl = slice(seg_ptr[s], seg_ptr[s + 1])
+        assert np.allclose(out.cpu().numpy()[:, sl], W[s].T @ V[:, sl], rtol=1e-11, atol=1e-10)
