@@ -51,6 +51,11 @@ SIGNATURES = {
     "inch_seg_combine": (_STATUS, [_i32, _vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
     "inch_seg_resnorm": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
                                    _vp]),
+    "inch_gnn_dense_fwd": (_STATUS, [_i64, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "inch_gnn_work_size": (_i64, [_i32, _i32]),
+    "inch_gnn_dense_bwd": (_STATUS, [_i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_gnn_aggregate": (_STATUS, [_i64, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp]),
+    "inch_gnn_relu_bwd": (_STATUS, [_i64, _vp, _vp, _vp, _vp]),
     "inch_rr_combine": (_STATUS, [_i64, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
 }
 

@@ -282,6 +282,7 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
     rel = rn.cpu().numpy().reshape(S, k) / lam_max
     order = np.argsort(seg_struct)
     done = np.where(np.isnan(done_at), total, done_at)
+    n_steps = ncv + (it + 1) * (ncv - k)        # operator applications of the lockstep Lanczos
     return BatchResult(eigval=Sel[order], rel_res=rel[order], latency=done[order], seconds=total,
-                       restarts=it + 1, n_spmv=op.n_spmv, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
+                       restarts=it + 1, n_spmv=n_steps, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
                        seg_struct=seg_struct, A=A, eigvec=Xr, converged=int(conv.sum()))

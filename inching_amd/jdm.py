@@ -124,7 +124,8 @@ class JDM:
         return mx
 
     def run(self, tol=1e-10, maxiter=1000, corr_maxiter=20, inter_tol=1e-3, gap=0.1, corr_factor=1e-4,
-            seed=0, v0=None, verbose=False):
+            seed=0, v0=None, V0=None, verbose=False):
+        """V0 (optional): rows [m, n] orthonormal, a warm start search space (m <= jmax - 1)."""
         t0 = time.time()
         rng = np.random.default_rng(seed)
         k, n = self.k, self.n
@@ -135,9 +136,21 @@ class JDM:
         else:
             v = torch.as_tensor(v0, dtype=torch.float64, device=V.device).reshape(-1).clone()
         v /= torch.linalg.norm(v)
-        V[0].copy_(v)
-        self.op.apply(V[0], self.Az)
-        G = np.array([[self._dot(V[0], self.Az)]])
+        if V0 is not None:
+            m0 = min(int(V0.shape[0]), self.jmax - 1)
+            V[:m0].copy_(V0[:m0])
+            Gd = torch.empty((m0, m0), dtype=torch.float64, device=V.device)
+            AV = torch.empty(n, dtype=torch.float64, device=V.device)
+            for j in range(m0):
+                self.op.apply(V[j], AV)
+                ops.multidot(V, m0, AV, self.h, self.ws)
+                Gd[:, j] = self.h[:m0]
+            G = Gd.cpu().numpy()
+            G = 0.5 * (G + G.T)
+        else:
+            V[0].copy_(v)
+            self.op.apply(V[0], self.Az)
+            G = np.array([[self._dot(V[0], self.Az)]])
         nconv, jacc = 0, 0
         theta, cur_tol = None, None
         it = 0

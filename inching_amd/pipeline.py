@@ -146,3 +146,32 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
                       ncv=min(max(factor * k, k + 32), A.shape[0] - 1),
                       degree=int(pol.AdjustedDegree) if pol is not None else 0,
                       seconds=time.time() - t_all, precision=precision)
+
+
+def lowest_modes_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
+    """lowest_modes started from the learned warm block: Rayleigh-Ritz of the predicted
+    block (float64, true operator), then TRLM from the combination of its k lowest Ritz
+    vectors. Falls back to the cold start when the warm solve misses the tolerance, so
+    the returned modes never depend on the network."""
+    from .warmstart import warm_block
+    t0 = time.time()
+    lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision)
+    bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
+    theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
+    t_warm = time.time() - t0
+    m = min(k, X.shape[0])
+    g = torch.Generator(device=A.device)
+    g.manual_seed(int(kw.get("seed", 0)) + 7)
+    c = torch.rand(m, generator=g, dtype=torch.float64, device=A.device) + 0.5
+    v0 = (c[:, None] * X[:m]).sum(0)
+    r = lowest_modes(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, v0=v0, **kw)
+    r.phases["warm_block"] = t_warm
+    r.seconds = time.time() - t0
+    r.warm_rel_res = info.get("rel_res")
+    r.fallback = False
+    if float(np.max(r.rel_res)) > tol_rel:
+        cold = lowest_modes(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, **kw)
+        cold.fallback = True
+        cold.seconds = time.time() - t0
+        return cold
+    return r

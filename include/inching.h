@@ -193,6 +193,23 @@ inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld
 inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, int64_t ld,
                             const double *W, double *out, int64_t ld_out, void *stream);
 
+/* ------------------------------------- learned warm start (graph network) ---- */
+/* No counterpart in the reference (its solvers start from random vectors). Layer widths
+ * <= 96; X/W/Y float16 row-major, accumulation and gradients float32. */
+/* Y = act(X W + b) for X [N, Din], W [Din, Dout]; Ypre (float32) and Y32 optional. */
+inch_status inch_gnn_dense_fwd(int64_t N, int32_t Din, int32_t Dout, const void *X, const void *W, const float *b,
+                               int32_t relu, float *Ypre, void *Y, float *Y32, void *stream);
+/* dX = dY W^T (dX may be NULL), dW = X^T dY, db = sum_i dY[i]; work: inch_gnn_work_size floats. */
+int64_t inch_gnn_work_size(int32_t Din, int32_t Dout);
+inch_status inch_gnn_dense_bwd(int64_t N, int32_t Din, int32_t Dout, const void *X, const void *W, const float *dY,
+                               float *dX, float *dW, float *db, float *work, void *stream);
+/* Neighbour mean over the ANM graph (self excluded): mode 0 Y[i] = mean_{j in N(i)} X[j];
+ * mode 1 its adjoint. dtype codes: 0 float32, 2 float16. */
+inch_status inch_gnn_aggregate(int64_t N, int32_t D, const int64_t *rowptr, const int32_t *col, const void *X,
+                               int32_t xdt, void *Y, int32_t ydt, int32_t mode, void *stream);
+/* dPre = dY where pre > 0 else 0. */
+inch_status inch_gnn_relu_bwd(int64_t n, const float *pre, const float *dY, float *dPre, void *stream);
+
 /* ---------------------------------------------- batched (segmented) FP64 ---- */
 /* Many structures stored as contiguous dof segments of one vector. Chunk c covers dofs
  * [cbeg[c], cend[c]) of segment cseg[c]; segment s owns chunks sptr[s] .. sptr[s+1]-1.

@@ -267,3 +267,22 @@ def test_cgs2_reorth_fused(cuda, J, n):
     assert np.allclose(ud.cpu().numpy(), u2, rtol=1e-10, atol=1e-12)
     assert np.allclose(Vd[J].cpu().numpy(), u2 / b_ref, rtol=1e-10, atol=1e-12)
     assert np.max(np.abs(V @ Vd[J].cpu().numpy())) < 1e-13
+
+
+def test_matfree_launch_configs_agree(mat):
+    from inching_amd import ops
+    from inching_amd.autotune import MATFREE_CONFIGS
+    A, Bs = mat
+    rng = np.random.default_rng(9)
+    x, z, acc0 = (rng.standard_normal(A.shape[0]) for _ in range(3))
+    ref = None
+    for cfg in MATFREE_CONFIGS:
+        y = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
+        acc = _dev(acc0)
+        ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, _dev(x), y, s1=0.3, c0=2.0, z=_dev(z), c2=1.0, acc=acc,
+                         mu=0.5, config=cfg)
+        out = 0.3 * (Bs @ x - 2.0 * x) - z
+        assert np.allclose(y.cpu().numpy(), out, rtol=1e-12, atol=1e-12), cfg
+        assert np.allclose(acc.cpu().numpy(), acc0 + 0.5 * out, rtol=1e-12, atol=1e-12)
+        if ref is None:
+            ref = y.cpu().numpy()

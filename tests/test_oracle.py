@@ -10,6 +10,7 @@ from scipy.sparse.csgraph import reverse_cuthill_mckee
 from oracle import chebyshev as OC
 from oracle import geometry as OG
 from oracle import hessian as OH
+from oracle import gnn as ON
 from oracle import lanczos as OL
 from oracle import partition as OP
 from oracle import rcm as OR
@@ -290,3 +291,32 @@ def test_irlm_matches_dense_eigh():
     lam, V, it = OL.irlm(lambda x: A @ x, A.shape[0], k, np.random.default_rng(0), tol=1e-10, Q=Q, shift=10.0,
                          maxiter=3000)
     assert np.allclose(lam, w[6:6 + k], atol=1e-7)
+
+
+# ---------------------------------------------------------- warm-start network --
+def test_oracle_layer_backward_finite_differences():
+    """Pins the oracle's hand-derived backward (float64, no rounding) by central differences."""
+    X = small_globule(120, seed=72)
+    ip, ix = OG.neighbour_csr(X, 8.0)
+    rng = np.random.default_rng(1)
+    h = rng.standard_normal((120, 4))
+    Ws, Wn, b = rng.standard_normal((4, 5)), rng.standard_normal((4, 5)), rng.standard_normal(5)
+    dout = rng.standard_normal((120, 5))
+    ident = lambda a: np.asarray(a, dtype=np.float64)  # noqa: E731
+    f16, ON.f16 = ON.f16, ident
+    try:
+        out, cache = ON.layer_forward(ip, ix, h, Ws, Wn, b)
+        dh, dWs, dWn, db = ON.layer_backward(ip, ix, cache, Ws, Wn, dout)
+        L = lambda *a: float(np.sum(ON.layer_forward(ip, ix, *a)[0] * dout))  # noqa: E731
+        e = 1e-6
+        for (arr, grad) in ((h, dh), (Ws, dWs), (Wn, dWn), (b, db)):
+            for idx in [(0,) * arr.ndim, tuple(s - 1 for s in arr.shape)]:
+                old = arr[idx]
+                arr[idx] = old + e
+                lp = L(h, Ws, Wn, b)
+                arr[idx] = old - e
+                lm = L(h, Ws, Wn, b)
+                arr[idx] = old
+                assert abs((lp - lm) / (2 * e) - grad[idx]) < 1e-5 * max(1.0, abs(grad[idx]))
+    finally:
+        ON.f16 = f16
