@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
+    ap.add_argument("--workload", default="globule", choices=["globule", "batch", "assembly", "warm"],
+                    help="globule: the headline line; batch / assembly / warm: the other configurations, one "
+                         "JSON line each")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
                     help="only build the matrix and launch each operator kernel REPS times (profiling runs)")
     return ap.parse_args()
@@ -88,6 +91,86 @@ def spmv_only(args):
     print(json.dumps(out))
 
 
+def _pct(a, q):
+    import numpy as np
+    return round(float(np.percentile(a, q)), 4)
+
+
+def run_batch(args):
+    """512 seeded globules, N log-uniform in [500, 20000] (seed 2), lowest 20 modes each, one
+    lockstep launch sequence (HIP-graph-captured restarts); solves/s and latency percentiles."""
+    import numpy as np
+    import torch
+
+    from inching_amd import synth
+    from inching_amd.batch import solve_batch
+    sizes = synth.batch_sizes(512, seed=2)
+    structs = [synth.globule(int(n), seed=1000 + i) for i, n in enumerate(sizes)]
+    solve_batch(structs[:16], k=20)                      # warm-up: code paging, allocator, autotune
+    torch.cuda.synchronize()
+    r = solve_batch(structs, k=20, precision="mf64", tol_rel=1e-8)
+    print(json.dumps({"metric": "batched lowest-20-modes solves/s (512 globules, 500-20000 atoms)",
+                      "value": round(512 / r.seconds, 3), "unit": "solves/s", "seconds": round(r.seconds, 3),
+                      "p50_latency_s": _pct(r.latency, 50), "p99_latency_s": _pct(r.latency, 99),
+                      "total_atoms": int(sizes.sum()), "restarts": r.restarts, "converged": r.converged,
+                      "max_rel_res": float(np.max(r.rel_res)), "ncv": r.ncv, "operator": "mf64",
+                      "lanczos_steps": r.n_spmv}))
+
+
+def run_assembly(args):
+    """60 x 40,000-atom subunits on an icosahedral orbit (radius 450 A), lowest 1000 modes."""
+    import numpy as np
+    import torch
+
+    from inching_amd import synth
+    from inching_amd.batch import solve_assembly
+    xyz, comp = synth.assembly(n_sub=60, atoms_per_sub=40000, shell_radius=450.0, seed0=100)
+    small = np.concatenate([xyz[comp == c] for c in range(3)])
+    solve_assembly(small, 30, per_component=14)           # warm-up
+    torch.cuda.synchronize()
+    r = solve_assembly(xyz, 1000, tol_rel=1e-8)
+    print(json.dumps({"metric": "time-to-1000-modes, 2.4M-atom icosahedral assembly (s)", "value": round(r.seconds, 3),
+                      "unit": "s", "components": r.components, "per_component_modes": r.per_component,
+                      "build_s": round(r.build_seconds, 3), "restarts": r.restarts, "filter_degree": r.degree,
+                      "max_rel_res": float(np.max(r.rel_res)), "lam_1000": float(r.eigval[-1]),
+                      "rigid_modes_deflated": 6 * r.components, "operator": "mf64"}))
+
+
+def run_warm(args):
+    """Learned warm start vs cold start on unseen 300k- and 1M-atom globules (32 modes)."""
+    import numpy as np
+    import torch
+
+    from inching_amd import synth, warmstart as W
+    from inching_amd.hessian import build_hessian, rigid_body_basis
+    from inching_amd.pipeline import lowest_modes, lowest_modes_warm
+    net = W.WarmStartNet.load()
+    out = {"metric": "learned warm start vs cold start: operator launches and time to rel_res 1e-8", "cases": []}
+    for n_atoms, seed in ((300000, 4), (1000000, 5)):
+        X = synth.globule(n_atoms, seed=seed)
+        A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=())
+        Q = rigid_body_basis(A.xyz)
+        lowest_modes(A, 8, Q=Q, precision="mf64")                 # warm-up at this size
+        torch.cuda.synchronize()
+        t = time.time()
+        cold = lowest_modes(A, 32, Q=Q, precision="mf64", tol_rel=1e-8)
+        torch.cuda.synchronize()
+        tc = time.time() - t
+        t = time.time()
+        warm = lowest_modes_warm(A, 32, net, Q=Q, precision="mf64", tol_rel=1e-8)
+        torch.cuda.synchronize()
+        tw = time.time() - t
+        out["cases"].append({"n_atoms": n_atoms, "cold_s": round(tc, 3), "warm_s": round(tw, 3),
+                             "cold_spmv": int(cold.total_spmv), "warm_spmv": int(warm.total_spmv),
+                             "warm_fallback": bool(warm.fallback),
+                             "max_rel_res_cold": float(np.max(cold.rel_res)),
+                             "max_rel_res_warm": float(np.max(warm.rel_res)),
+                             "max_abs_dlambda": float(np.max(np.abs(warm.eigval - cold.eigval)))})
+        print("bench: warm case %d done" % n_atoms, file=sys.stderr, flush=True)
+        del A, Q
+    print(json.dumps(out))
+
+
 def kernel_bytes(kind, nnzb, n_atoms, epi):
     """Algorithmic bytes of one operator launch (values + vectors; no index bytes)."""
     nd = 3 * n_atoms
@@ -108,6 +191,8 @@ def main():
         sys.exit(2)
     if args.spmv_only:
         return spmv_only(args)
+    if args.workload != "globule":
+        return {"batch": run_batch, "assembly": run_assembly, "warm": run_warm}[args.workload](args)
     import numpy as np
     import torch
 

@@ -142,7 +142,7 @@ class BatchResult:
 
 
 def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
-                factor=2, seed=0, use_graph=True, verbose=False):
+                factor=2, seed=0, use_graph=True, verbose=False, pol=None):
     """Lowest k non-trivial modes of every structure (list of [N_s, 3] coordinate arrays, A).
 
     Returns BatchResult with per-structure eigenvalues [S, k] (PlusI included), eigenvectors
@@ -162,7 +162,42 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
     seg_atoms = np.r_[starts, A.n_atoms].astype(np.int64)
     if len(starts) != len(structures):
         raise RuntimeError("structures are not contiguous after ordering (are they closer than the cutoff?)")
-    S = len(structures)
+    return solve_segments(A, seg_atoms, k, seg_struct=seg_struct, precision=precision, tol_rel=tol_rel, shift=shift,
+                          maxiter=maxiter, factor=factor, seed=seed, use_graph=use_graph, verbose=verbose, t0=t0,
+                          pol=pol)
+
+
+def components(A):
+    """Connected components of the neighbour graph of A as contiguous atom ranges of the
+    RCM order: returns the component start offsets [C+1] (the BFS of the ordering visits
+    each component completely before the next one, so every component is contiguous)."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components
+    rp = A.rowptr.cpu().numpy()
+    col = A.col.cpu().numpy()
+    n = A.n_atoms
+    G = csr_matrix((np.ones(col.size, dtype=np.int8), col, rp), shape=(n, n))
+    ncomp, lab = connected_components(G, directed=False)
+    starts = np.nonzero(np.r_[True, lab[1:] != lab[:-1]])[0]
+    if len(starts) != ncomp:
+        raise RuntimeError("components are not contiguous in this order")
+    return np.r_[starts, n].astype(np.int64)
+
+
+def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
+                   factor=2, seed=0, use_graph=True, verbose=False, t0=None, pol=None, ptol=1e-10):
+    """Lockstep thick-restart Lanczos over the contiguous atom segments of a block-diagonal
+    A: k modes per segment, per-segment rigid-body Hotelling deflation, optional common
+    Chebyshev filter ``pol`` (largest k of p(A) per segment, then Rayleigh quotients)."""
+    dev = A.device
+    if t0 is None:
+        torch.cuda.synchronize()
+        t0 = time.time()
+    if seg_struct is None:
+        seg_struct = np.arange(len(seg_atoms) - 1)
+    if pol is not None and shift > 0:
+        shift = -40.0
+    S = len(seg_atoms) - 1
     n = A.shape[0]
     seg = Segments(3 * seg_atoms, dev)
     min_dof = int(seg.sizes.min())
@@ -171,7 +206,8 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
         raise AssertionError("ABORTED. k must be smaller than every structure's dof count minus the rigid modes")
     so = SegOps(seg, max(ncv, 6) + 1, dev)
     Q = segmented_rigid_basis(A.xyz, seg_atoms, so)
-    op = Operator(A, precision)
+    op = Operator(A, precision, pol=pol)
+    filtered = pol is not None
     f64 = dict(dtype=torch.float64, device=dev)
     V = torch.empty((ncv, n), **f64)
     Vk = torch.empty((k, n), **f64)
@@ -222,7 +258,10 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
             T[:, k, :k] = bk
             T[:, :k, k] = bk
         w, Z = np.linalg.eigh(T)                 # ascending per structure
-        Sel, W = w[:, :k], Z[:, :, :k]
+        if filtered:
+            Sel, W = w[:, ::-1][:, :k], Z[:, :, ::-1][:, :, :k]
+        else:
+            Sel, W = w[:, :k], Z[:, :, :k]
         Wd = torch.as_tensor(np.ascontiguousarray(W), device=dev)
         so.combine(V, ncv, Wd, Vk)
         V[:k].copy_(Vk)
@@ -244,8 +283,8 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
     it = 0
     # lam_max for the relative residual: a 300-step Lanczos bound of the whole batch
     from .chebyshev import spectrum_bound
-    lam_max = spectrum_bound(A, precision=precision)[1]
-    tol_abs = tol_rel * lam_max
+    lam_max = spectrum_bound(A, precision=precision)[1] if pol is None else float(pol.cc + pol.dd)
+    tol_abs = tol_rel * lam_max if pol is None else ptol * np.sqrt(k)
     conv = res <= tol_abs
     done_at[conv] = time.time() - t0
     for it in range(maxiter):
@@ -274,15 +313,74 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
     Xr = V[:k]
     Yc = A.matfree(Xr.T.contiguous())
     Y = Yc.T.contiguous()
+    if filtered:
+        # Rayleigh quotients with A per segment and mode (the Ritz values above are of p(A))
+        rq = torch.empty((k, S), **f64)
+        for q in range(k):
+            so.multidot(Xr[q:q + 1], 1, Y[q], rq[q])
+        Sel = rq.T.cpu().numpy()                  # [S, k], paired with the rows of Xr
     theta = torch.as_tensor(np.ascontiguousarray(Sel), device=dev)
     rn = torch.empty(S * k, **f64)
     so.resnorm(Xr, Y, theta, rn)
     torch.cuda.synchronize()
     total = time.time() - t0
     rel = rn.cpu().numpy().reshape(S, k) / lam_max
+    if filtered:
+        lam_max = float(pol.cc + pol.dd)
+        rel = rn.cpu().numpy().reshape(S, k) / lam_max
+        ordq = np.argsort(Sel, axis=1)             # ascending eigenvalues per segment
+        Sel = np.take_along_axis(Sel, ordq, axis=1)
+        rel = np.take_along_axis(rel, ordq, axis=1)
     order = np.argsort(seg_struct)
     done = np.where(np.isnan(done_at), total, done_at)
     n_steps = ncv + (it + 1) * (ncv - k)        # operator applications of the lockstep Lanczos
     return BatchResult(eigval=Sel[order], rel_res=rel[order], latency=done[order], seconds=total,
                        restarts=it + 1, n_spmv=n_steps, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
                        seg_struct=seg_struct, A=A, eigvec=Xr, converged=int(conv.sum()))
+
+
+def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, per_component=None,
+                   filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False):
+    """Lowest ``k_total`` non-trivial modes of a structure made of disconnected components
+    (e.g. subunits of an assembly farther apart than the cutoff). H is block-diagonal, so its
+    spectrum is the union of the components' spectra: every component is solved for
+    ``per_component`` modes in one lockstep run (each with its own six rigid-body modes
+    deflated), the results are merged, and the merge is checked: a component whose
+    highest computed mode lies below the k_total-th merged eigenvalue could hide more modes
+    below it, which raises (increase ``per_component``)."""
+    from . import chebyshev as CH
+    ops.require_gpu()
+    dev = torch.device("cuda", torch.cuda.current_device())
+    torch.cuda.synchronize()
+    t0 = time.time()
+    prec = {"mf64": (), "f32": ("f32",), "f64": ("f64",)}[precision]
+    A = build_hessian(xyz, rc, plus_i=plus_i, reorder=True, precision=prec, device=dev)
+    seg_atoms = components(A)
+    C = len(seg_atoms) - 1
+    t_build = time.time() - t0
+    ks = int(per_component) if per_component else int(np.ceil(1.4 * k_total / C)) + 4
+    pol = None
+    if filtered:
+        lo, hi = CH.spectrum_bound(A, precision=precision)
+        bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
+        pol, cnt, interval, _ = CH.optimize_on_memory(A, bound, A.plus_i, wanted_count=int(C * (ks + 6) * 1.5),
+                                                      designated_start=bound[1] / 10.0, max_degree=max_degree,
+                                                      min_degree=5, margin=max(5, C * ks // 10), trials=5,
+                                                      ratio=0.7, seed=seed, precision=precision)
+    r = solve_segments(A, seg_atoms, ks, precision=precision, tol_rel=tol_rel, maxiter=maxiter, seed=seed,
+                       verbose=verbose, t0=t0, pol=pol)
+    flat = r.eigval.reshape(-1)
+    order = np.argsort(flat, kind="stable")[:k_total]
+    cutoff = flat[order[-1]]
+    short = [s for s in range(C) if r.eigval[s, -1] <= cutoff]
+    if short:
+        raise RuntimeError("components %s may have more than %d modes below the %d-th eigenvalue; "
+                           "raise per_component" % (short[:10], ks, k_total))
+    comp_of = order // ks
+    mode_of = order % ks
+    torch.cuda.synchronize()
+    return BatchResult(eigval=flat[order], component=comp_of, mode_in_component=mode_of,
+                       rel_res=r.rel_res.reshape(-1)[order], seconds=time.time() - t0, build_seconds=t_build,
+                       components=C, per_component=ks, restarts=r.restarts, n_spmv=r.n_spmv,
+                       degree=int(pol.AdjustedDegree) if pol is not None else 0, A=A, seg_atoms=seg_atoms,
+                       eigvec=r.eigvec, lam_max=r.lam_max)

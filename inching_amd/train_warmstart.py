@@ -50,7 +50,7 @@ def main(argv=None):
     ap.add_argument("--min-atoms", type=int, default=1000)
     ap.add_argument("--max-atoms", type=int, default=20000)
     ap.add_argument("--modes", type=int, default=32)
-    ap.add_argument("--channels", type=int, default=48)
+    ap.add_argument("--channels", type=int, default=32)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=3)

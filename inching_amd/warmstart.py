@@ -11,7 +11,7 @@ accumulation; every layer's forward and backward runs in the HIP kernels of
 * per-atom input features from the centred coordinates scaled by the structure radius
   (position, radius, the 6 quadratic monomials, neighbour count);
 * layer l: h <- relu(h W_self + mean_{j in N(i)} h_j W_nbr + b) over the ANM neighbour graph;
-* head: 64 -> 3C (C vector fields per atom), i.e. a block P [3N, C].
+* head: 64 -> 3C (C vector fields per atom), i.e. a block P [3N, C] (C <= 32: the head emits 3C <= 96 channels).
 
 Loss (subspace, so it is blind to the sign/rotation ambiguity of eigenvectors inside
 near-degenerate clusters): with U the solver's lowest K non-trivial modes (orthonormal)
@@ -148,7 +148,7 @@ class Head(torch.autograd.Function):
 
 
 class WarmStartNet:
-    def __init__(self, channels=48, seed=0, device=None):
+    def __init__(self, channels=32, seed=0, device=None):
         self.C = int(channels)
         dev = device or torch.device("cuda", torch.cuda.current_device())
         g = torch.Generator(device="cpu")
@@ -167,7 +167,7 @@ class WarmStartNet:
         self.params += [Wo, bo]
 
     def forward(self, xyz, rowptr, col):
-        """Block P [3N, C] (float32) for coordinates in the matrix's atom order."""
+        """Block P [3N, C] (C <= 32: the head emits 3C <= 96 channels).(float32) for coordinates in the matrix's atom order."""
         gph = _Graph(rowptr, col)
         h = features(xyz, rowptr)
         for l in range(LAYERS):
@@ -198,7 +198,7 @@ class WarmStartNet:
 
 
 def subspace_loss(P, U, Q=None):
-    """1 - tr(U^T P (P^T P)^-1 P^T U) / K  (U [3N, K] orthonormal, P [3N, C])."""
+    """1 - tr(U^T P (P^T P)^-1 P^T U) / K  (U [3N, K] orthonormal, P [3N, C] (C <= 32: the head emits 3C <= 96 channels).."""
     P = P.to(torch.float64)
     if Q is not None:
         P = P - Q.T @ (Q @ P)

@@ -76,6 +76,33 @@ def test_rcm_matches_scipy_reverse_cuthill_mckee(seed):
     assert np.array_equal(order, ref)
 
 
+@pytest.mark.parametrize("case", ["globule", "two_blobs", "scipy_seed", "single_atom"])
+def test_c_twin_matches_numpy_oracle(case):
+    """The plain-C restatement (cell grid + BFS) and the numpy one (k-d tree) give the same
+    neighbour CSR and the same permutation; with numpy's default argsort as the seed order the
+    C twin equals scipy's reverse_cuthill_mckee too."""
+    from oracle import c_twin as CT
+    rng = np.random.default_rng(11)
+    if case == "single_atom":
+        X = np.zeros((1, 3))
+    elif case == "two_blobs":
+        X = np.concatenate([rng.uniform(0, 15, (300, 3)), rng.uniform(60, 70, (120, 3))])
+    else:
+        X = small_globule(1500, seed=5)
+    a = OG.neighbour_csr(X, 8.0)
+    b = CT.neighbour_csr(X, 8.0)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    seed_order = np.argsort(np.diff(a[0])) if case == "scipy_seed" else None
+    o1, u1 = OR.rcm_order(*a, seed_order=seed_order)
+    o2, u2 = CT.rcm_order(*b, seed_order=seed_order)
+    assert np.array_equal(o1, o2) and np.array_equal(u1, u2)
+    if case == "scipy_seed":
+        n = len(a[0]) - 1
+        ixd = np.concatenate([a[1][a[0][i]:a[0][i + 1]][::-1] for i in range(n)])
+        G = sp.csr_matrix((np.ones(len(ixd)), ixd, a[0]), shape=(n, n))
+        assert np.array_equal(o2, reverse_cuthill_mckee(G, symmetric_mode=True))
+
+
 def test_rcm_disconnected_components_and_bandwidth():
     rng = np.random.default_rng(3)
     A = rng.uniform(0, 15, size=(200, 3))
