@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
+    ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--workload", default="globule", choices=["globule", "batch", "assembly", "warm"],
                     help="globule: the headline line; batch / assembly / warm: the other configurations, one "
                          "JSON line each")
@@ -64,13 +65,16 @@ def spmv_only(args):
     z32, y32, acc32 = torch.randn_like(x32), torch.empty_like(x32), torch.zeros_like(x32)
     x64 = torch.randn(n, dtype=torch.float64, device="cuda")
     z64, y64, acc64 = torch.randn_like(x64), torch.empty_like(x64), torch.zeros_like(x64)
+    from inching_amd.autotune import best_config
+    cfg = {p: best_config(A, p) for p in ("f32", "f64", "mf64")}       # the launch shapes the solver uses
+    out["kernels"] = {p: kernel_key(p, A) for p in cfg}
     cases = {
         "bsr_f32_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x32, y32, s1=0.1, c0=2.0, z=z32,
-                                             c2=1.0, acc=acc32, mu=0.01, lanes=A.lanes),
+                                             c2=1.0, acc=acc32, mu=0.01, lanes=cfg["f32"]),
         "bsr_f64_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, x64, y64, s1=0.1, c0=2.0, z=z64,
-                                             c2=1.0, acc=acc64, mu=0.01, lanes=A.lanes),
+                                             c2=1.0, acc=acc64, mu=0.01, lanes=cfg["f64"]),
         "matfree_f64_cheb": lambda: ops.matfree_spmv(A.xyz, A.rowptr, A.col, 1.0, x64, y64, s1=0.1, c0=2.0, z=z64,
-                                                     c2=1.0, acc=acc64, mu=0.01),
+                                                     c2=1.0, acc=acc64, mu=0.01, config=cfg["mf64"]),
     }
     for name, fn in cases.items():
         for _ in range(3):
@@ -118,7 +122,9 @@ def run_batch(args):
 
 
 def run_assembly(args):
-    """60 x 40,000-atom subunits on an icosahedral orbit (radius 450 A), lowest 1000 modes."""
+    """60 x 40,000-atom subunits on an icosahedral orbit (radius 450 A), lowest 1000 modes:
+    the matrix-free float64 operator (headline) and the stored float64 block operator (the
+    plain solver), same filtered lockstep TRLM over the 60 components."""
     import numpy as np
     import torch
 
@@ -128,12 +134,25 @@ def run_assembly(args):
     small = np.concatenate([xyz[comp == c] for c in range(3)])
     solve_assembly(small, 30, per_component=14)           # warm-up
     torch.cuda.synchronize()
-    r = solve_assembly(xyz, 1000, tol_rel=1e-8)
-    print(json.dumps({"metric": "time-to-1000-modes, 2.4M-atom icosahedral assembly (s)", "value": round(r.seconds, 3),
-                      "unit": "s", "components": r.components, "per_component_modes": r.per_component,
-                      "build_s": round(r.build_seconds, 3), "restarts": r.restarts, "filter_degree": r.degree,
-                      "max_rel_res": float(np.max(r.rel_res)), "lam_1000": float(r.eigval[-1]),
-                      "rigid_modes_deflated": 6 * r.components, "operator": "mf64"}))
+    out = {"metric": "time-to-1000-modes, 2.4M-atom icosahedral assembly (s)", "unit": "s"}
+    runs = {}
+    for prec in ["mf64"] + [m for m in args.compare.split(",") if m in ("f64", "f32")]:
+        print("bench: assembly %s start" % prec, file=sys.stderr, flush=True)
+        r = solve_assembly(xyz, 1000, precision=prec, tol_rel=1e-8, verbose=args.verbose)
+        runs[prec] = r
+        out[prec] = {"seconds": round(r.seconds, 3), "build_s": round(r.build_seconds, 3), "restarts": r.restarts,
+                     "filter_degree": r.degree, "lanczos_steps": int(r.n_spmv),
+                     "max_rel_res": float(np.max(r.rel_res)), "lam_1000": float(r.eigval[-1])}
+        print("bench: assembly %s done in %.2f s" % (prec, r.seconds), file=sys.stderr, flush=True)
+        del r
+        torch.cuda.empty_cache()
+    h = runs["mf64"]
+    out.update({"value": out["mf64"]["seconds"], "components": h.components, "per_component_modes": h.per_component,
+                "rigid_modes_deflated": 6 * h.components, "operator": "mf64"})
+    if "f64" in runs:
+        out["speedup_vs_plain_f64"] = round(out["f64"]["seconds"] / out["mf64"]["seconds"], 3)
+        out["max_abs_dlambda_vs_f64"] = float(np.max(np.abs(runs["f64"].eigval - h.eigval)))
+    print(json.dumps(out))
 
 
 def run_warm(args):
@@ -181,6 +200,36 @@ def kernel_bytes(kind, nnzb, n_atoms, epi):
         return 9 * 8 * nnzb + (5 if epi == "cheb" else 2) * nd * 8
     # matrix-free: neighbour indices + coordinates + vectors
     return 4 * nnzb + 8 * (n_atoms + 1) + 24 * n_atoms + (5 if epi == "cheb" else 2) * nd * 8
+
+
+KERNEL_DESC = {"f32": "block SpMV, fp32 values, Chebyshev epilogue",
+               "f64": "block SpMV, fp64 values, Chebyshev epilogue",
+               "mf64": "matrix-free fp64 ANM product, Chebyshev epilogue"}
+
+
+def kernel_key(kind, A):
+    """Template name of the operator kernel the autotuner picked for this matrix."""
+    from inching_amd.autotune import best_config
+    cfg = best_config(A, kind)
+    if kind == "mf64":
+        return "k_matfree_epi<double,%d,%d>" % (cfg // 10, cfg % 10)
+    t = "float" if kind == "f32" else "double"
+    return "k_bsr_spmv<%s,%s,%d>" % (t, t, cfg)
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of ``key`` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (calibrated on a copy kernel), or None when that kernel was not counted."""
+    path = os.path.join(ROOT, "profiles", "spmv_kernels_globule300k_pmc_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 3 and p[0] == "traffic" and p[1] == key:
+                    return int(p[2])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -249,10 +298,8 @@ def main():
         by = kernel_bytes(kind, nnzb, n_atoms, epi)
         ach = by / (avg_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": {"f32": "k_bsr_spmv<float,float,16> (block SpMV, fp32 values, Chebyshev epilogue)",
-                           "f64": "k_bsr_spmv<double,double,16> (block SpMV, fp64 values, Chebyshev epilogue)",
-                           "mf64": "k_matfree_epi<double,16> (matrix-free fp64 ANM product, Chebyshev epilogue)"}[kind],
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_key(kind, A)),
+                "kernel": kernel_key(kind, A) + " (" + KERNEL_DESC[kind] + ")",
                 "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // steps,
                 "bytes_per_launch": int(by), "kernel_share_of_step": round(ms_total / steps / 1e3 / step_s, 3)}
 

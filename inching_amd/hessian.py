@@ -149,20 +149,26 @@ def _to_device_xyz(X, device):
     return t.contiguous()
 
 
-def rcm_order(X, rc, seed_order=None, reverse=True, device=None):
-    """3DRCM permutation of coordinates X: (order int32, undo int64) numpy arrays."""
+def rcm_order(X, rc, seed_order=None, reverse=True, device=None, rcm="device"):
+    """3DRCM permutation of coordinates X: (order int32, undo int64) numpy arrays.
+    ``rcm``: 'device' (level-synchronous BFS kernels) or 'host' (sequential C++); both give
+    the same permutation."""
     ops.require_gpu()
     device = device or torch.device("cuda", torch.cuda.current_device())
     xyz = _to_device_xyz(X, device)
     indptr, indices = ops.neighbours(xyz, rc)
+    if rcm == "device":
+        order, undo = ops.rcm_device(indptr, indices, seed_order=seed_order, reverse=reverse)
+        return order.cpu().numpy(), undo.cpu().numpy()
     return ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order, reverse=reverse)
 
 
-def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_order=None, device=None):
+def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_order=None, device=None, rcm="device"):
     """Coordinates -> BlockHessian (in RCM order when ``reorder``).
 
     ``precision``: which value copies to assemble ('f32', 'f64'). Returns the container;
-    ``A.order`` / ``A.undo`` hold the permutation (identity when not reordered).
+    ``A.order`` / ``A.undo`` hold the permutation (identity when not reordered). ``rcm``:
+    'device' or 'host' ordering (the same permutation).
     """
     ops.require_gpu()
     device = device or torch.device("cuda", torch.cuda.current_device())
@@ -170,9 +176,14 @@ def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_orde
     n = xyz.shape[0]
     if reorder:
         indptr, indices = ops.neighbours(xyz, rc)
-        order, undo = ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order)
+        if rcm == "device":
+            order_d, undo_d = ops.rcm_device(indptr, indices, seed_order=seed_order)
+            xyz = xyz[order_d.long()].contiguous()
+            order, undo = order_d.cpu().numpy(), undo_d.cpu().numpy()
+        else:
+            order, undo = ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order)
+            xyz = xyz[torch.as_tensor(order.astype(np.int64), device=device)].contiguous()
         del indptr, indices
-        xyz = xyz[torch.as_tensor(order.astype(np.int64), device=device)].contiguous()
     else:
         order = np.arange(n, dtype=np.int32)
         undo = np.arange(n, dtype=np.int64)

@@ -72,6 +72,32 @@ def rcm_host(indptr, indices, seed_order=None, reverse=True):
     return order, undo
 
 
+def rcm_device(indptr, indices, seed_order=None, reverse=True, stream=None):
+    """3DRCM on the device: (order int32 [n], undo int64 [n]) device tensors, bit-exact to
+    rcm_host. indptr int64 / indices int32 device tensors (unique columns per row).
+    seed_order: optional seed order (default: stable argsort of the degree, computed on the
+    host from the row pointers with a radix sort)."""
+    n = indptr.numel() - 1
+    dev = indptr.device
+    if n <= 0:
+        raise ValueError("empty graph")
+    if seed_order is None:
+        deg = np.diff(indptr.cpu().numpy())
+        key = deg.astype(np.int16) if deg.max(initial=0) < 32767 else deg
+        seeds = torch.as_tensor(np.argsort(key, kind="stable").astype(np.int64), device=dev)
+    else:
+        seeds = torch.as_tensor(np.ascontiguousarray(seed_order, dtype=np.int64), device=dev).reshape(-1)
+        if seeds.numel() != n:
+            raise ValueError("seed_order must have shape (n,)")
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    undo = torch.empty(n, dtype=torch.int64, device=dev)
+    wb = int(call("inch_rcm_device_work_size", n))
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    call("inch_rcm_device", n, P(indptr), P(indices), P(seeds), 1 if reverse else 0, P(order), P(undo), P(work), wb,
+         stream_ptr(stream))
+    return order, undo
+
+
 def batch_head(n, maxleaf):
     cap = 2 * (int(n) // max(1, int(maxleaf) // 2) + 2) + 8
     head = np.empty(cap, dtype=np.int64)

@@ -17,7 +17,8 @@
  * Reference interfaces replaced (paths relative to the reference repository):
  *  inch_neighbors_celllist   scipy cKDTree.query_ball_point calls of
  *                            InchingLiteInteger/Fuel/Coordinate/T1.py:219,239,276-277,365,397
- *  inch_rcm_host             X_KdCuthillMckeeOrder, Fuel/Coordinate/T1.py:171-323
+ *  inch_rcm_host, inch_rcm_device
+ *                            X_KdCuthillMckeeOrder, Fuel/Coordinate/T1.py:171-323
  *  inch_batch_head           GetPartitionTree/FlattenPartitionTree util.py:563-587 and
  *                            Fuel/T1.py:145-153
  *  inch_hessian_assemble     Xnumpy_SparseCupyMatrixUngappped.ReturnCupyHLowerTriangleInt32/
@@ -76,6 +77,18 @@ inch_status inch_synth_globule(int64_t n, double radius, double dmin, uint64_t s
 inch_status inch_rcm_host(int64_t n, const int64_t *indptr, const int32_t *indices,
                           const int64_t *seed_order, int32_t reverse, int32_t *order,
                           int64_t *undo);
+
+/* The same permutation computed on the device (level-synchronous BFS, bit-exact to
+ * inch_rcm_host; replaces X_KdCuthillMckeeOrder, Fuel/Coordinate/T1.py:171-323).
+ * indptr / indices / seeds / order / undo are device pointers; rows hold unique column
+ * indices (the neighbour CSR). seeds: int64[n] seed order, a permutation (the stable
+ * argsort of the degree reproduces inch_rcm_host's default). work: device scratch of
+ * inch_rcm_device_work_size(n) bytes. Syncs the stream (one level size per BFS level
+ * comes back to the host). Inputs are validated on the device before use. */
+int64_t inch_rcm_device_work_size(int64_t n);
+inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *indices, const int64_t *seeds,
+                            int32_t reverse, int32_t *order, int64_t *undo, void *work, int64_t work_bytes,
+                            void *stream);
 
 /* Row-batch heads of recursive halving with leaves <= maxleaf, sorted descending
  * and cumulated: head[0..nbatch]. cap = capacity of head. Host only. */

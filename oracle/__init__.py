@@ -18,6 +18,10 @@ rcm.rcm_order                 scipy.sparse.csgraph.reverse_cuthill_mckee on the
                               seed order; brute-force enumeration of all valid
                               Cuthill-McKee orders on tiny graphs; C twin
                               (oracle/c/rcm_oracle.c) agrees bit-for-bit.
+rcm.rcm_order_levels          level-synchronous form (the device algorithm):
+                              equal to rcm_order on random geometric and
+                              non-geometric graphs, both seed orders, both
+                              directions.
 partition.batch_head          closed form of recursive halving (leaf sizes in
                               (L/2, L], sum = N) on many N.
 partition.ungapped_islands    brute-force definition check (every neighbour

@@ -63,6 +63,40 @@ def rcm_order(indptr, indices, seed_order=None, reverse=True):
     return out.astype(np.int32), np.argsort(out).astype(np.int64)
 
 
+def rcm_order_levels(indptr, indices, seed_order=None, reverse=True):
+    """Level-synchronous form of :func:`rcm_order` (the order the device kernels build):
+    level L+1 = unvisited neighbours of level L, each owned by the first level-L node
+    adjacent to it, sorted by (owner position, degree, -index). Equal to rcm_order on every
+    graph; kept as a restatement to check that equivalence on the CPU."""
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices, dtype=np.int64)
+    n = indptr.shape[0] - 1
+    degree = np.diff(indptr)
+    inds = np.argsort(degree, kind="stable") if seed_order is None else np.asarray(seed_order, dtype=np.int64)
+    pos = np.full(n, -1, dtype=np.int64)
+    order = []
+    for seed in inds:
+        if pos[seed] >= 0:
+            continue
+        pos[seed] = len(order)
+        order.append(seed)
+        level = [seed]
+        while level:
+            owner = {}
+            for p_node in level:
+                p = pos[p_node]
+                for j in indices[indptr[p_node]:indptr[p_node + 1]]:
+                    if pos[j] < 0 and (j not in owner or owner[j] > p):
+                        owner[j] = p
+            nxt = sorted(owner, key=lambda j: (owner[j], degree[j], -j))
+            for j in nxt:
+                pos[j] = len(order)
+                order.append(j)
+            level = nxt
+    out = np.asarray(order[::-1] if reverse else order, dtype=np.int64)
+    return out.astype(np.int32), np.argsort(out).astype(np.int64)
+
+
 def bandwidth(indptr, indices, order):
     """Max |p(i) - p(j)| over neighbour pairs after permuting by ``order``."""
     indptr = np.asarray(indptr, dtype=np.int64)

@@ -69,6 +69,51 @@ def test_rcm_pipeline_matches_oracle(glob2k):
     assert np.array_equal(o, o_ref) and np.array_equal(u, u_ref)
 
 
+@pytest.mark.parametrize("case", ["globule", "blobs_and_singletons", "numpy_default_seeds", "forward", "40k"])
+def test_rcm_device_bit_exact(glob2k, glob40k, case):
+    """The level-synchronous device RCM gives the oracle's permutation (and, at 40k atoms,
+    the sequential host implementation's), for several components and isolated atoms, an
+    explicit seed order and the unreversed order."""
+    from inching_amd import ops
+    _print("case: device rcm %s" % case)
+    rng = np.random.default_rng(5)
+    if case == "blobs_and_singletons":
+        X = np.concatenate([rng.uniform(0, 15, (300, 3)), rng.uniform(60, 70, (150, 3)),
+                            np.array([[200.0, 0, 0], [-200.0, 0, 0], [0, 300.0, 0]])])
+    elif case == "40k":
+        X = glob40k
+    else:
+        X = glob2k
+    xyz = _dev(X)
+    ip_d, ix_d = ops.neighbours(xyz, 8.0)
+    ip, ix = ip_d.cpu().numpy(), ix_d.cpu().numpy()
+    seeds = np.argsort(np.diff(ip)) if case == "numpy_default_seeds" else None
+    reverse = case != "forward"
+    o, u = ops.rcm_device(ip_d, ix_d, seed_order=seeds, reverse=reverse)
+    o, u = o.cpu().numpy(), u.cpu().numpy()
+    if case == "40k":
+        o_ref, u_ref = ops.rcm_host(ip, ix)
+    else:
+        o_ref, u_ref = OR.rcm_order(ip, ix, seed_order=seeds, reverse=reverse)
+    assert np.array_equal(o, o_ref) and np.array_equal(u, u_ref)
+
+
+def test_rcm_device_rejects_bad_input(glob2k):
+    from inching_amd import ops
+    from inching_amd._lib import InchingError
+    _print("case: device rcm bad input")
+    ip_d, ix_d = ops.neighbours(_dev(glob2k), 8.0)
+    n = ip_d.numel() - 1
+    bad = np.arange(n)
+    bad[5] = n + 7                                   # out of range: caught before any indexed access
+    with pytest.raises(InchingError):
+        ops.rcm_device(ip_d, ix_d, seed_order=bad)
+    dup = np.arange(n)
+    dup[5] = dup[6]                                  # not a permutation
+    with pytest.raises(InchingError):
+        ops.rcm_device(ip_d, ix_d, seed_order=dup)
+
+
 def test_assembly_matches_oracle(glob2k):
     from inching_amd.hessian import build_hessian
     _print("case: assembly n=2000")

@@ -103,6 +103,82 @@ def test_c_twin_matches_numpy_oracle(case):
         assert np.array_equal(o2, reverse_cuthill_mckee(G, symmetric_mode=True))
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_level_synchronous_rcm_equals_sequential(seed):
+    """The level-synchronous restatement (owner = first adjacent node of the previous level,
+    then degree, then descending index) reproduces the sequential order, on random
+    geometric graphs with several components and on random non-geometric graphs."""
+    rng = np.random.default_rng(seed)
+    if seed % 2:
+        n = 300
+        rows = [np.unique(np.concatenate([[i], rng.integers(0, n, rng.integers(0, 6))])) for i in range(n)]
+        A = sp.lil_matrix((n, n))
+        for i, r in enumerate(rows):
+            A[i, r] = 1
+            A[r, i] = 1
+        A = A.tocsr()
+        A.sort_indices()
+        ip, ix = A.indptr.astype(np.int64), A.indices.astype(np.int64)
+    else:
+        X = np.concatenate([rng.uniform(0, 20, (250, 3)), rng.uniform(50, 60, (80, 3)), rng.uniform(-90, -80, (3, 3))])
+        ip, ix = OG.neighbour_csr(X, 5.0)
+    for so in (None, np.argsort(np.diff(ip))):
+        for rev in (True, False):
+            a = OR.rcm_order(ip, ix, seed_order=so, reverse=rev)
+            b = OR.rcm_order_levels(ip, ix, seed_order=so, reverse=rev)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def _all_cuthill_mckee_orders(adj, deg):
+    """Every order a Cuthill-McKee BFS can produce when ties are broken arbitrarily: a new
+    component starts at an unvisited node of minimal degree, and each dequeued node appends
+    its unvisited neighbours in non-decreasing degree (any order within equal degree)."""
+    import itertools
+    n = len(adj)
+    out = set()
+
+    def rec(order, ptr):
+        if len(order) == n:
+            out.add(tuple(order))
+            return
+        if ptr < len(order):
+            node = order[ptr]
+            kids = sorted({j for j in adj[node] if j not in order})
+            groups = [list(g) for _, g in itertools.groupby(sorted(kids, key=lambda j: deg[j]), key=lambda j: deg[j])]
+            for combo in itertools.product(*[list(itertools.permutations(g)) for g in groups]):
+                rec(order + [j for g in combo for j in g], ptr + 1)
+            return
+        rest = [j for j in range(n) if j not in order]
+        dmin = min(deg[j] for j in rest)
+        for sd in rest:
+            if deg[sd] == dmin:
+                rec(order + [sd], ptr)
+
+    rec([], 0)
+    return out
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_rcm_is_a_cuthill_mckee_order_brute_force(seed):
+    """On tiny random graphs the (unreversed) order is one of the enumerated Cuthill-McKee
+    orders, i.e. level structure, degree sorting and seed choice are right independently of
+    the tie rule."""
+    rng = np.random.default_rng(100 + seed)
+    n = int(rng.integers(4, 8))
+    adj = [{i} for i in range(n)]
+    for _ in range(int(rng.integers(2, 9))):
+        a, b = rng.integers(0, n, 2)
+        adj[a].add(int(b))
+        adj[b].add(int(a))
+    ip = np.zeros(n + 1, dtype=np.int64)
+    ip[1:] = np.cumsum([len(r) for r in adj])
+    ix = np.concatenate([sorted(r) for r in adj]).astype(np.int64)
+    deg = np.diff(ip)
+    valid = _all_cuthill_mckee_orders(adj, deg)
+    order, _ = OR.rcm_order(ip, ix, reverse=False)
+    assert tuple(order.tolist()) in valid
+
+
 def test_rcm_disconnected_components_and_bandwidth():
     rng = np.random.default_rng(3)
     A = rng.uniform(0, 15, size=(200, 3))
