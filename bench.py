@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--warm-atoms", default="300000,1000000", help="globule sizes of --workload warm")
     ap.add_argument("--workload", default="globule", choices=["globule", "batch", "assembly", "warm"],
                     help="globule: the headline line; batch / assembly / warm: the other configurations, one "
                          "JSON line each")
@@ -73,8 +74,8 @@ def spmv_only(args):
                                              c2=1.0, acc=acc32, mu=0.01, lanes=cfg["f32"]),
         "bsr_f64_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, x64, y64, s1=0.1, c0=2.0, z=z64,
                                              c2=1.0, acc=acc64, mu=0.01, lanes=cfg["f64"]),
-        "matfree_f64_cheb": lambda: ops.matfree_spmv(A.xyz, A.rowptr, A.col, 1.0, x64, y64, s1=0.1, c0=2.0, z=z64,
-                                                     c2=1.0, acc=acc64, mu=0.01, config=cfg["mf64"]),
+        "matfree_f64_cheb": lambda: A.matfree_spmv(x64, y64, s1=0.1, c0=2.0, z=z64, c2=1.0, acc=acc64, mu=0.01,
+                                                   config=cfg["mf64"]),
     }
     for name, fn in cases.items():
         for _ in range(3):
@@ -92,6 +93,8 @@ def spmv_only(args):
         ops.stream_copy(src, dst)
     torch.cuda.synchronize()
     out["stream_copy_bytes_read"] = src.numel() * 8
+    from inching_amd.autotune import table
+    out["autotune_ms"] = {k: v["ms"] for k, v in table().items()}
     print(json.dumps(out))
 
 
@@ -156,37 +159,45 @@ def run_assembly(args):
 
 
 def run_warm(args):
-    """Learned warm start vs cold start on unseen 300k- and 1M-atom globules (32 modes)."""
+    """Learned warm start vs cold start on unseen 300k- and 1M-atom globules (32 modes):
+    thick-restart Lanczos started from a combination of the warm block, and Jacobi-Davidson
+    started from the whole warm block; operator launches and time to rel_res 1e-8."""
     import numpy as np
     import torch
 
     from inching_amd import synth, warmstart as W
     from inching_amd.hessian import build_hessian, rigid_body_basis
-    from inching_amd.pipeline import lowest_modes, lowest_modes_warm
+    from inching_amd.pipeline import lowest_modes, lowest_modes_jdm, lowest_modes_jdm_warm, lowest_modes_warm
     net = W.WarmStartNet.load()
     out = {"metric": "learned warm start vs cold start: operator launches and time to rel_res 1e-8", "cases": []}
-    for n_atoms, seed in ((300000, 4), (1000000, 5)):
+    sizes = [int(s) for s in args.warm_atoms.split(",")]
+    for n_atoms, seed in zip(sizes, (4, 5, 6, 7)):
         X = synth.globule(n_atoms, seed=seed)
         A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=())
         Q = rigid_body_basis(A.xyz)
         lowest_modes(A, 8, Q=Q, precision="mf64")                 # warm-up at this size
-        torch.cuda.synchronize()
-        t = time.time()
-        cold = lowest_modes(A, 32, Q=Q, precision="mf64", tol_rel=1e-8)
-        torch.cuda.synchronize()
-        tc = time.time() - t
-        t = time.time()
-        warm = lowest_modes_warm(A, 32, net, Q=Q, precision="mf64", tol_rel=1e-8)
-        torch.cuda.synchronize()
-        tw = time.time() - t
-        out["cases"].append({"n_atoms": n_atoms, "cold_s": round(tc, 3), "warm_s": round(tw, 3),
-                             "cold_spmv": int(cold.total_spmv), "warm_spmv": int(warm.total_spmv),
-                             "warm_fallback": bool(warm.fallback),
-                             "max_rel_res_cold": float(np.max(cold.rel_res)),
-                             "max_rel_res_warm": float(np.max(warm.rel_res)),
-                             "max_abs_dlambda": float(np.max(np.abs(warm.eigval - cold.eigval)))})
-        print("bench: warm case %d done" % n_atoms, file=sys.stderr, flush=True)
-        del A, Q
+        case = {"n_atoms": n_atoms}
+        runs = {}
+        for name, fn in (("trlm_cold", lambda: lowest_modes(A, 32, Q=Q, precision="mf64", tol_rel=1e-8)),
+                         ("trlm_warm", lambda: lowest_modes_warm(A, 32, net, Q=Q, precision="mf64", tol_rel=1e-8)),
+                         ("jdm_cold", lambda: lowest_modes_jdm(A, 32, Q=Q, precision="mf64", tol_rel=1e-8)),
+                         ("jdm_warm", lambda: lowest_modes_jdm_warm(A, 32, net, Q=Q, precision="mf64",
+                                                                    tol_rel=1e-8))):
+            torch.cuda.synchronize()
+            t = time.time()
+            r = fn()
+            torch.cuda.synchronize()
+            runs[name] = r
+            case[name] = {"seconds": round(time.time() - t, 3), "operator_launches": int(r.total_spmv),
+                          "max_rel_res": float(np.max(r.rel_res)), "fallback": bool(getattr(r, "fallback", False))}
+            print("bench: warm case %d %s done" % (n_atoms, name), file=sys.stderr, flush=True)
+        ref = runs["trlm_cold"].eigval
+        for name in ("trlm_warm", "jdm_cold", "jdm_warm"):
+            case[name]["max_abs_dlambda_vs_trlm_cold"] = float(np.max(np.abs(runs[name].eigval - ref)))
+        case["jdm_launch_ratio_warm_over_cold"] = round(runs["jdm_warm"].total_spmv / runs["jdm_cold"].total_spmv, 3)
+        out["cases"].append(case)
+        del A, Q, runs
+        torch.cuda.empty_cache()
     print(json.dumps(out))
 
 

@@ -82,8 +82,8 @@ def best_config(A, precision, reps=5):
     times = {}
     if kind == "matfree":
         for cfg in MATFREE_CONFIGS:
-            times[cfg] = _time(lambda: ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, x, y, s1=0.1, c0=1.0,
-                                                        z=z, c2=1.0, acc=acc, mu=0.1, config=cfg), reps)
+            times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
+                               reps)
     else:
         vals = A.vals(precision)
         for lanes in BSR_LANES:

@@ -347,7 +347,7 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
 
     def av(x, y):
         if vals is None:
-            ops.matfree_spmv(A.xyz, A.rowptr, A.col, A.plus_i, x, y, config=cfg)
+            A.matfree_spmv(x, y, config=cfg)
         else:
             ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=cfg)
 

@@ -184,7 +184,9 @@ template <int C>
 __global__ void __launch_bounds__(256) k_cgs_pass_b(int64_t len, int J, const double *__restrict__ V, int64_t ld,
                                                      const double *__restrict__ h1, double *__restrict__ u,
                                                      double *__restrict__ part) {
-  constexpr int LPC = 256 / C;  // lanes per column
+  constexpr int LPC = 256 / C;        // lanes per column
+  constexpr int RS = 256 / C;         // rows staged per pass of the workgroup
+  constexpr int NR = (4096 / C) / RS; // staging loads per thread (J <= 4096 / C)
   __shared__ double vs[4096 / C][C + 1];
   __shared__ double hs[kCgsMaxJ];
   __shared__ double up[C];
@@ -192,14 +194,27 @@ __global__ void __launch_bounds__(256) k_cgs_pass_b(int64_t len, int J, const do
   for (int t = tid; t < J; t += 256) hs[t] = h1[t];
   double acc = 0.0, ss = 0.0;
   const int64_t ntiles = (len + C - 1) / C;
+  // the next tile's V[:J, tile] is fetched into registers while the current one is used
+  const int c_ld = tid % C, r_ld = tid / C;
+  double pre[NR];
+  auto fetch = [&](int64_t tile) {
+    const int64_t i = tile * C + c_ld;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int r = r_ld + q * RS;
+      pre[q] = (r < J && i < len) ? V[(int64_t)r * ld + i] : 0.0;
+    }
+  };
+  if (blockIdx.x < ntiles) fetch(blockIdx.x);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int64_t c0 = tile * C;
     __syncthreads();
-    {
-      const int c = tid % C;
-      const int64_t i = c0 + c;
-      for (int r = tid / C; r < J; r += 256 / C) vs[r][c] = (i < len) ? V[(int64_t)r * ld + i] : 0.0;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int r = r_ld + q * RS;
+      if (r < J) vs[r][c_ld] = pre[q];
     }
+    if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
     __syncthreads();
     {
       const int c = tid / LPC, pl = tid % LPC;

@@ -92,6 +92,11 @@ class BlockHessian:
         ops.bsr_spmm(self.n_atoms, self.rowptr, self.col, self.vals(precision), X.contiguous(), Y)
         return Y
 
+    def matfree_spmv(self, x, y, config=0, **epi):
+        """Matrix-free float64 product with the fused epilogue (``config``: autotuner code
+        lanes*10+unroll, 0 = default)."""
+        ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config, **epi)
+
     def matfree(self, X):
         """FP64 (H + PlusI I) X recomputed from coordinates (X [3n] or [3n, r])."""
         Y = torch.empty_like(X)

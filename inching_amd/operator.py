@@ -67,7 +67,7 @@ class Operator:
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
         if self.vals is None:
-            ops.matfree_spmv(self.A.xyz, self.A.rowptr, self.A.col, self.A.plus_i, x, y, config=self.lanes, **epi)
+            self.A.matfree_spmv(x, y, config=self.lanes, **epi)
         else:
             ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.lanes, **epi)
         if self.timer is not None:

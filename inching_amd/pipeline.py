@@ -175,3 +175,54 @@ def lowest_modes_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
         cold.seconds = time.time() - t0
         return cold
     return r
+
+
+def lowest_modes_jdm(A, k, Q=None, precision="mf64", tol_rel=1e-8, V0=None, shift=40.0, factor=2, maxiter=5000,
+                     seed=0, bound=None):
+    """k lowest non-trivial eigenpairs by Jacobi-Davidson on A (+ shift Q^T Q), the method of
+    ``Burn/JacobiDavidsonHotellingDeflation/T1.py:382-757``, optionally started from the
+    search space V0 (rows). Accepted on the float64 residual after a float64 Rayleigh-Ritz
+    polish, like ``lowest_modes``."""
+    from .jdm import JDM
+    t0 = time.time()
+    n_spmv = {}
+    if bound is None:
+        lo, hi = CH.spectrum_bound(A, precision=precision)
+        bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
+        n_spmv["bound"] = 301
+    lam_max = bound[1]
+    op = Operator(A, precision, Q=Q, shift=shift)
+    t = time.time()
+    r = JDM(op, k, factor=factor).run(tol=0.5 * tol_rel * lam_max, maxiter=maxiter, V0=V0, seed=seed)
+    phases = {"solve_jdm_" + precision: time.time() - t}
+    n_spmv["jdm"] = op.n_spmv
+    theta, X, info = polish(A, r.eigvec, Q=Q, lam_max=lam_max)
+    torch.cuda.synchronize()
+    return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(info["rel_res"]), lam_max=lam_max,
+                      phases=phases, n_spmv=n_spmv, total_spmv=op.n_spmv, restarts=int(r.iters), n_apply=op.n_apply,
+                      ncv=factor * k, degree=0, seconds=time.time() - t0, precision=precision,
+                      converged=int(r.converged))
+
+
+def lowest_modes_jdm_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
+    """lowest_modes_jdm started from the learned warm block (its float64 Ritz vectors as the
+    initial search space). Falls back to the cold start when the warm solve misses the
+    tolerance, so the returned modes never depend on the network."""
+    from .warmstart import warm_block
+    t0 = time.time()
+    lo, hi = CH.spectrum_bound(A, precision=precision)
+    bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
+    theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
+    factor = int(kw.get("factor", 2))
+    V0 = X[:min(X.shape[0], factor * k - 1)].contiguous()
+    r = lowest_modes_jdm(A, k, Q=Q, precision=precision, tol_rel=tol_rel, V0=V0, bound=bound, **kw)
+    r.phases["warm_block"] = time.time() - t0 - r.seconds
+    r.seconds = time.time() - t0
+    r.warm_rel_res = info.get("rel_res")
+    r.fallback = False
+    if r.converged < k or float(np.max(r.rel_res)) > tol_rel:
+        cold = lowest_modes_jdm(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, **kw)
+        cold.fallback = True
+        cold.seconds = time.time() - t0
+        return cold
+    return r

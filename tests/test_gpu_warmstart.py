@@ -70,20 +70,26 @@ def test_training_reduces_subspace_loss(cuda):
                                                     "weights", "warmstart_gnn.npz")),
                     reason="trained weights not present")
 def test_trained_network_saves_matvecs(cuda):
-    """On an unseen globule the warm start needs fewer operator applications than the cold
-    start for the same verified tolerance, and returns the same eigenvalues."""
+    """On an unseen globule the learned warm block, used as the initial search space of
+    Jacobi-Davidson, reaches the same verified tolerance with fewer operator applications than
+    the cold start, and both return the eigenvalues of the cold thick-restart Lanczos solve.
+    (A single start vector built from the block gives TRLM no measurable gain: its restart
+    count is set by the filter, not by the start.)"""
     from inching_amd import synth, warmstart as W
     from inching_amd.hessian import build_hessian, rigid_body_basis
-    from inching_amd.pipeline import lowest_modes, lowest_modes_warm
+    from inching_amd.pipeline import lowest_modes, lowest_modes_jdm, lowest_modes_jdm_warm
     print("case: warm vs cold on a 30k-atom globule", flush=True)
     net = W.WarmStartNet.load()
     X = synth.globule(30000, seed=991)
     A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=())
     Q = rigid_body_basis(A.xyz)
     k = 32
-    cold = lowest_modes(A, k, Q=Q, precision="mf64", tol_rel=1e-8)
-    warm = lowest_modes_warm(A, k, net, Q=Q, precision="mf64", tol_rel=1e-8)
+    ref = lowest_modes(A, k, Q=Q, precision="mf64", tol_rel=1e-8)
+    cold = lowest_modes_jdm(A, k, Q=Q, precision="mf64", tol_rel=1e-8)
+    warm = lowest_modes_jdm_warm(A, k, net, Q=Q, precision="mf64", tol_rel=1e-8)
+    print("jdm cold %d matvecs, warm %d" % (cold.total_spmv, warm.total_spmv), flush=True)
     assert not warm.fallback
     assert np.max(warm.rel_res) <= 1e-8 and np.max(cold.rel_res) <= 1e-8
-    assert np.allclose(warm.eigval, cold.eigval, rtol=0, atol=1e-9)
-    assert warm.total_spmv < cold.total_spmv, (warm.total_spmv, cold.total_spmv)
+    assert np.allclose(warm.eigval, ref.eigval, rtol=0, atol=1e-9)
+    assert np.allclose(cold.eigval, ref.eigval, rtol=0, atol=1e-9)
+    assert warm.total_spmv < 0.9 * cold.total_spmv, (warm.total_spmv, cold.total_spmv)
