@@ -9,6 +9,10 @@ like the reference's lower-triangle CSR for the operations callers use (``A @ v`
 solver of this package consumes directly. ``A_diag`` is the float64 diagonal of H + PlusI I
 (a torch tensor on the GPU).
 
+``User_DictCharmmGuiPbc`` (a CHARMM-GUI box dictionary, nm) adds the periodic x/y images of
+reference :98-128, :317-396: each block sums -d d^T / |d|^2 over the images of j within the
+cutoff.
+
 As in the reference, X (nm) is expected in the order the caller wants the matrix in
 (normally after ``X_KdCuthillMckeeOrder``); the builder does not reorder. The HalfNnz
 buffer-size argument is checked like the reference's preallocated buffers: a matrix with
@@ -24,9 +28,11 @@ from inching_amd.hessian import build_hessian
 class Xnumpy_SparseCupyMatrixUngappped:
     def __init__(self, X, batch_head=None, maxleafsize=100, rc_Gamma=8.0, device=None, User_PlusI=1.0,
                  dtype_temp=torch.float64, X_precision=None, NnzMinMaxDict=None, User_DictCharmmGuiPbc=None):
-        if User_DictCharmmGuiPbc is not None:
-            raise NotImplementedError("periodic images (User_DictCharmmGuiPbc) are not supported by this build")
         ops.require_gpu()
+        self.box, self.origin = None, (0.0, 0.0)
+        if User_DictCharmmGuiPbc is not None:
+            from InchingLiteInteger.Fuel.Coordinate.T1 import pbc_box
+            self.box, self.origin = pbc_box(User_DictCharmmGuiPbc, rc_Gamma / 10.0)
         self.X = X
         self.n_atoms = int(X.shape[0])
         self.rc_Gamma = rc_Gamma / 10.0
@@ -47,7 +53,7 @@ class Xnumpy_SparseCupyMatrixUngappped:
 
     def _build(self, User_MaxHalfNnzBufferSize, precision):
         A = build_hessian(self.X, self.rc_Gamma, plus_i=self.User_PlusI, reorder=False, precision=precision,
-                          device=self.device)
+                          device=self.device, box=self.box, origin=self.origin)
         half = (A.nnzb - A.n_atoms) // 2 * 9 + A.n_atoms * 6
         if User_MaxHalfNnzBufferSize is not None and half > int(User_MaxHalfNnzBufferSize) + 1:
             raise IndexError("the lower triangle holds %d scalars, more than User_MaxHalfNnzBufferSize=%d"

@@ -43,7 +43,7 @@ def parse():
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--warm-atoms", default="300000,1000000", help="globule sizes of --workload warm")
-    ap.add_argument("--workload", default="globule", choices=["globule", "batch", "assembly", "warm"],
+    ap.add_argument("--workload", default="globule", choices=["globule", "batch", "assembly", "warm", "cpu_plain"],
                     help="globule: the headline line; batch / assembly / warm: the other configurations, one "
                          "JSON line each")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
@@ -158,6 +158,55 @@ def run_assembly(args):
     print(json.dumps(out))
 
 
+def run_cpu_plain(args):
+    """The plain method on the CPU (5,000-atom globule, seed 0, 64 modes, tol 1e-9, FP64 CSR
+    Hessian, RCM order, rigid-body Hotelling deflation): the oracle's TRLM on the host beside
+    this library's solve of the same matrix on the GPU, both checked against dense eigh."""
+    import numpy as np
+    import scipy.sparse as sp
+    import torch
+
+    from inching_amd import synth
+    from inching_amd.hessian import build_hessian, rigid_body_basis
+    from inching_amd.pipeline import lowest_modes
+    from oracle import geometry as OG
+    from oracle import hessian as OH
+    from oracle import lanczos as OL
+    from oracle import rcm as OR
+    X = synth.globule(5000, seed=0)
+    t = time.time()
+    ip, ix = OG.neighbour_csr(X, 8.0)
+    order, _ = OR.rcm_order(ip, ix)
+    Xp = X[order]
+    rp, col, v = OH.block_hessian(Xp, 8.0, 1.0)
+    H = sp.bsr_matrix((v, col.astype(np.int64), rp), shape=(15000, 15000)).tocsr()
+    Qc = OH.rigid_body_basis(Xp)
+    t_build = time.time() - t
+    t = time.time()
+    lam_c, V_c, it_c = OL.trlm(lambda x: H @ x, 15000, 64, np.random.default_rng(0), tol=1e-9, Q=Qc, shift=10.0)
+    t_cpu = time.time() - t
+    print("bench: cpu trlm done in %.1f s" % t_cpu, file=sys.stderr, flush=True)
+    A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=("f64",))
+    Q = rigid_body_basis(A.xyz)
+    lowest_modes(A, 64, Q=Q, precision="f64", tol_rel=1e-9)
+    torch.cuda.synchronize()
+    t = time.time()
+    r = lowest_modes(A, 64, Q=Q, precision="f64", tol_rel=1e-9)
+    torch.cuda.synchronize()
+    t_gpu = time.time() - t
+    t = time.time()
+    w = np.linalg.eigvalsh(H.toarray())
+    t_eigh = time.time() - t
+    want = w[6:70]
+    print(json.dumps({"metric": "plain method, 5,000-atom globule, 64 modes, tol 1e-9", "unit": "s",
+                      "cpu_oracle_trlm_s": round(t_cpu, 3), "cpu_oracle_build_s": round(t_build, 3),
+                      "cpu_cores": len(os.sched_getaffinity(0)), "gpu_f64_solve_s": round(t_gpu, 4),
+                      "dense_eigh_s": round(t_eigh, 2),
+                      "cpu_max_abs_dlambda_vs_eigh": float(np.max(np.abs(lam_c - want))),
+                      "gpu_max_abs_dlambda_vs_eigh": float(np.max(np.abs(r.eigval - want))),
+                      "gpu_max_rel_res": float(np.max(r.rel_res)), "cpu_restarts": int(it_c)}))
+
+
 def run_warm(args):
     """Learned warm start vs cold start on unseen 300k- and 1M-atom globules (32 modes):
     thick-restart Lanczos started from a combination of the warm block, and Jacobi-Davidson
@@ -252,7 +301,8 @@ def main():
     if args.spmv_only:
         return spmv_only(args)
     if args.workload != "globule":
-        return {"batch": run_batch, "assembly": run_assembly, "warm": run_warm}[args.workload](args)
+        return {"batch": run_batch, "assembly": run_assembly, "warm": run_warm,
+                "cpu_plain": run_cpu_plain}[args.workload](args)
     import numpy as np
     import torch
 

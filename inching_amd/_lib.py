@@ -32,7 +32,12 @@ SIGNATURES = {
     "inch_anm_matfree_spmv": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp,
                                         _dbl, _vp, _vp, _i32, _i32, _vp]),
     "inch_rcm_device_work_size": (_i64, [_i64]),
-    "inch_rcm_device": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),
+    "inch_rcm_device": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp]),
+    "inch_neighbors_celllist_pbc": (_STATUS, [_i64, _vp, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp, _i64, _vp,
+                                              _vp]),
+    "inch_hessian_assemble_pbc": (_STATUS, [_i64, _vp, _dbl, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp]),
+    "inch_anm_matfree_pbc": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _i32, _vp, _vp, _i32, _dbl, _dbl,
+                                       _vp, _dbl, _vp, _dbl, _vp, _vp, _i32, _vp]),
     "inch_stream_copy": (_STATUS, [_i64, _vp, _vp, _vp]),
     "inch_multidot_work_size": (_i64, [_i32]),
     "inch_multidot": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),

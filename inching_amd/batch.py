@@ -169,19 +169,12 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
 
 def components(A):
     """Connected components of the neighbour graph of A as contiguous atom ranges of the
-    RCM order: returns the component start offsets [C+1] (the BFS of the ordering visits
-    each component completely before the next one, so every component is contiguous)."""
-    from scipy.sparse import csr_matrix
-    from scipy.sparse.csgraph import connected_components
-    rp = A.rowptr.cpu().numpy()
-    col = A.col.cpu().numpy()
-    n = A.n_atoms
-    G = csr_matrix((np.ones(col.size, dtype=np.int8), col, rp), shape=(n, n))
-    ncomp, lab = connected_components(G, directed=False)
-    starts = np.nonzero(np.r_[True, lab[1:] != lab[:-1]])[0]
-    if len(starts) != ncomp:
-        raise RuntimeError("components are not contiguous in this order")
-    return np.r_[starts, n].astype(np.int64)
+    RCM order: the component start offsets [C+1]. The device ordering visits each component
+    completely before the next one and reports where each starts."""
+    comps = getattr(A, "components", None)
+    if comps is None:
+        raise ValueError("components need a matrix built with reorder=True and the device ordering")
+    return np.asarray(comps, dtype=np.int64)
 
 
 def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,

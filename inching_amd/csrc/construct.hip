@@ -147,6 +147,102 @@ __global__ void k_neigh(int64_t n, const double *__restrict__ xyz, Grid g, doubl
   if (!indices) counts[i] = cnt;
 }
 
+// Periodic images in x and y (membrane boxes): image s = (sx, sy) in {-1,0,1}^2 shifts atom j
+// by (sx * bx, sy * by, 0); bit (sx + 1) * 3 + (sy + 1) of an entry's mask marks an image
+// within the cutoff, d_s = (x_j + s box) - x_i. Cells are periodic in x and y (cell size
+// >= rc, coordinates wrapped into the box from its origin) and open in z.
+struct PbcGrid {
+  double lo[3];      // box origin in x, y; minimum z
+  double cell[3];    // cell sizes
+  double box[2];
+  int64_t dim[3];
+};
+
+__device__ __forceinline__ int64_t pcell(double v, double lo, double box, double cell, int64_t dim) {
+  double w = v - lo;
+  w -= box * floor(w / box);
+  int64_t c = (int64_t)floor(w / cell);
+  return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
+}
+
+__device__ __forceinline__ int64_t zcell(double v, const PbcGrid &g) {
+  int64_t c = (int64_t)floor((v - g.lo[2]) / g.cell[2]);
+  return c < 0 ? 0 : (c >= g.dim[2] ? g.dim[2] - 1 : c);
+}
+
+__device__ __forceinline__ uint32_t image_mask(const double *__restrict__ xyz, int64_t i, int64_t j, double bx,
+                                               double by, double rc2) {
+  const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
+  const double dz = __dsub_rn(xyz[3 * j + 2], zi);
+  uint32_t m = 0;
+  for (int sx = -1; sx <= 1; ++sx) {
+    const double dx = __dsub_rn(__dadd_rn(xyz[3 * j], sx * bx), xi);
+    for (int sy = -1; sy <= 1; ++sy) {
+      const double dy = __dsub_rn(__dadd_rn(xyz[3 * j + 1], sy * by), yi);
+      if (dist2(dx, dy, dz) <= rc2) m |= 1u << ((sx + 1) * 3 + (sy + 1));
+    }
+  }
+  return m;
+}
+
+__global__ void k_cell_count_pbc(int64_t n, const double *__restrict__ xyz, PbcGrid g, int32_t *__restrict__ cell_of,
+                                 int32_t *__restrict__ count) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c0 = pcell(xyz[3 * i], g.lo[0], g.box[0], g.cell[0], g.dim[0]);
+  const int64_t c1 = pcell(xyz[3 * i + 1], g.lo[1], g.box[1], g.cell[1], g.dim[1]);
+  const int64_t c2 = zcell(xyz[3 * i + 2], g);
+  const int64_t id = (c0 * g.dim[1] + c1) * g.dim[2] + c2;
+  cell_of[i] = (int32_t)id;
+  atomicAdd(&count[id], 1);
+}
+
+// One thread per atom; every cell is visited once even when a periodic dimension has fewer
+// than three cells; an entry is kept when any image is within the cutoff.
+__global__ void k_neigh_pbc(int64_t n, const double *__restrict__ xyz, PbcGrid g, double rc2,
+                            const int32_t *__restrict__ start, const int32_t *__restrict__ atoms,
+                            const int64_t *__restrict__ indptr, int64_t *__restrict__ counts,
+                            int32_t *__restrict__ indices) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c0 = pcell(xyz[3 * i], g.lo[0], g.box[0], g.cell[0], g.dim[0]);
+  const int64_t c1 = pcell(xyz[3 * i + 1], g.lo[1], g.box[1], g.cell[1], g.dim[1]);
+  const int64_t c2 = zcell(xyz[3 * i + 2], g);
+  const int64_t na = g.dim[0] < 3 ? g.dim[0] : 3, nbb = g.dim[1] < 3 ? g.dim[1] : 3;
+  int64_t cnt = 0;
+  int64_t out = indices ? indptr[i] : 0;
+  const int64_t end = indices ? indptr[i + 1] : 0;
+  for (int64_t ia = 0; ia < na; ++ia) {
+    const int64_t a = g.dim[0] < 3 ? ia : (c0 - 1 + ia + g.dim[0]) % g.dim[0];
+    for (int64_t ib = 0; ib < nbb; ++ib) {
+      const int64_t b = g.dim[1] < 3 ? ib : (c1 - 1 + ib + g.dim[1]) % g.dim[1];
+      for (int64_t c = c2 - 1; c <= c2 + 1; ++c) {
+        if (c < 0 || c >= g.dim[2]) continue;
+        const int64_t id = (a * g.dim[1] + b) * g.dim[2] + c;
+        for (int32_t p = start[id]; p < start[id + 1]; ++p) {
+          const int32_t j = atoms[p];
+          if (image_mask(xyz, i, j, g.box[0], g.box[1], rc2)) {
+            if (indices) {
+              if (out < end) indices[out] = j;
+              ++out;
+            }
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  if (!indices) counts[i] = cnt;
+}
+
+__global__ void k_image_masks(int64_t n, const double *__restrict__ xyz, double bx, double by, double rc2,
+                              const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+                              uint16_t *__restrict__ images) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) images[e] = (uint16_t)image_mask(xyz, i, indices[e], bx, by, rc2);
+}
+
 // exclusive scan of int64 counts -> indptr (single block)
 __global__ void k_scan_i64(int64_t m, const int64_t *__restrict__ in, int64_t *__restrict__ out) {
   __shared__ int64_t buf[1024];
@@ -211,6 +307,7 @@ __global__ void __launch_bounds__(256) k_sort_rows(int64_t n, const int64_t *__r
 // block is minus the sum of the row's off-diagonal blocks plus plus_i * I.
 __global__ void __launch_bounds__(256) k_assemble(int64_t n, const double *__restrict__ xyz, double plus_i,
                                                    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                                                   const uint16_t *__restrict__ images, double bx, double by,
                                                    double *__restrict__ v64, float *__restrict__ v32,
                                                    double *__restrict__ diag) {
   const int lane = threadIdx.x & 63;
@@ -227,13 +324,31 @@ __global__ void __launch_bounds__(256) k_assemble(int64_t n, const double *__res
       selfpos = b;
       continue;
     }
-    double dx = __dsub_rn(xyz[3 * (int64_t)j], xi), dy = __dsub_rn(xyz[3 * (int64_t)j + 1], yi),
-           dz = __dsub_rn(xyz[3 * (int64_t)j + 2], zi);
-    double r2 = dist2(dx, dy, dz);
-    double c = r2 > 0.0 ? -1.0 / r2 : 0.0;
-    double d[3] = {dx, dy, dz};
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q) h[3 * p + q] = d[p] * d[q] * c;
+    if (!images) {
+      double dx = __dsub_rn(xyz[3 * (int64_t)j], xi), dy = __dsub_rn(xyz[3 * (int64_t)j + 1], yi),
+             dz = __dsub_rn(xyz[3 * (int64_t)j + 2], zi);
+      double r2 = dist2(dx, dy, dz);
+      double c = r2 > 0.0 ? -1.0 / r2 : 0.0;
+      double d[3] = {dx, dy, dz};
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) h[3 * p + q] = d[p] * d[q] * c;
+    } else {
+      // sum over the images of j within the cutoff
+      for (int t = 0; t < 9; ++t) h[t] = 0.0;
+      const uint32_t m = images[b];
+      const double dz = __dsub_rn(xyz[3 * (int64_t)j + 2], zi);
+      for (int bit = 0; bit < 9; ++bit) {
+        if (!((m >> bit) & 1u)) continue;
+        const double sx = (double)(bit / 3 - 1), sy = (double)(bit % 3 - 1);
+        const double dx = __dsub_rn(__dadd_rn(xyz[3 * (int64_t)j], sx * bx), xi);
+        const double dy = __dsub_rn(__dadd_rn(xyz[3 * (int64_t)j + 1], sy * by), yi);
+        const double r2 = dist2(dx, dy, dz);
+        const double c = r2 > 0.0 ? -1.0 / r2 : 0.0;
+        const double d[3] = {dx, dy, dz};
+        for (int p = 0; p < 3; ++p)
+          for (int q = 0; q < 3; ++q) h[3 * p + q] += d[p] * d[q] * c;
+      }
+    }
     s[0] -= h[0];
     s[1] -= h[1];
     s[2] -= h[2];
@@ -367,7 +482,111 @@ extern "C" inch_status inch_hessian_assemble(int64_t n, const double *xyz, doubl
   hipStream_t st = (hipStream_t)stream;
   const int bs = 256;
   const int64_t grid = (n * 64 + bs - 1) / bs;
-  k_assemble<<<grid, bs, 0, st>>>(n, xyz, plus_i, rowptr, col, vals64, vals32, diag);
+  k_assemble<<<grid, bs, 0, st>>>(n, xyz, plus_i, rowptr, col, nullptr, 0.0, 0.0, vals64, vals32, diag);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+extern "C" inch_status inch_neighbors_celllist_pbc(int64_t n, const double *xyz, double rc, double box_x,
+                                                   double box_y, double origin_x, double origin_y, int64_t *indptr,
+                                                   int32_t *indices, uint16_t *images, int64_t capacity,
+                                                   int64_t *nnz_out, void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && indptr && rc > 0, "inch_neighbors_celllist_pbc: bad args");
+  INCH_REQUIRE(box_x > rc && box_y > rc, "ABORTED. The PBC box size must exceed the cutoff (%g, %g vs %g)", box_x,
+               box_y, rc);
+  INCH_REQUIRE(n < (int64_t)1 << 31, "inch_neighbors_celllist_pbc: n must fit int32");
+  INCH_REQUIRE(!indices || images, "inch_neighbors_celllist_pbc: the fill pass needs images");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = 256;
+  DevBuf part;
+  HIP_TRY(hipMalloc(&part.p, nb * 6 * sizeof(double)));
+  k_minmax<<<nb, 256, 0, st>>>(n, xyz, (double *)part.p);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> hp(nb * 6);
+  HIP_TRY(hipMemcpyAsync(hp.data(), part.p, nb * 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  double zlo = INFINITY, zhi = -INFINITY;
+  for (int b = 0; b < nb; ++b) {
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(hp[b * 6 + a]) && hp[b * 6 + a] != INFINITY)
+        INCH_FAIL(INCH_ERR_NAN, "inch_neighbors_celllist_pbc: non-finite coordinates");
+    zlo = std::min(zlo, hp[b * 6 + 2]);
+    zhi = std::max(zhi, hp[b * 6 + 5]);
+  }
+  if (!std::isfinite(zlo) || !std::isfinite(zhi)) INCH_FAIL(INCH_ERR_NAN, "inch_neighbors_celllist_pbc: non-finite coordinates");
+  PbcGrid g;
+  g.lo[0] = origin_x;
+  g.lo[1] = origin_y;
+  g.lo[2] = zlo;
+  g.box[0] = box_x;
+  g.box[1] = box_y;
+  g.dim[0] = std::max<int64_t>(1, (int64_t)std::floor(box_x / rc));
+  g.dim[1] = std::max<int64_t>(1, (int64_t)std::floor(box_y / rc));
+  g.cell[0] = box_x / (double)g.dim[0];
+  g.cell[1] = box_y / (double)g.dim[1];
+  g.cell[2] = rc;
+  g.dim[2] = std::max<int64_t>(1, (int64_t)std::floor((zhi - zlo) / rc) + 1);
+  const int64_t ncell = g.dim[0] * g.dim[1] * g.dim[2];
+  if (ncell > ((int64_t)1 << 30)) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist_pbc: %lld cells", (long long)ncell);
+  DevBuf cell_of, cnt, start, atoms, counts, ovf;
+  HIP_TRY(hipMalloc(&cell_of.p, n * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&cnt.p, (ncell + 1) * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&start.p, (ncell + 1) * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&atoms.p, n * sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
+  const int bs = 256;
+  const int64_t gb = (n + bs - 1) / bs;
+  k_cell_count_pbc<<<gb, bs, 0, st>>>(n, xyz, g, (int32_t *)cell_of.p, (int32_t *)cnt.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_i32<<<1, 1024, 0, st>>>(ncell, (int32_t *)cnt.p, (int32_t *)start.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
+  k_cell_fill<<<gb, bs, 0, st>>>(n, (int32_t *)cell_of.p, (int32_t *)start.p, (int32_t *)cnt.p, (int32_t *)atoms.p);
+  HIP_TRY(hipGetLastError());
+  const double rc2 = rc * rc;
+  if (indices == nullptr) {
+    HIP_TRY(hipMalloc(&counts.p, n * sizeof(int64_t)));
+    k_neigh_pbc<<<gb, bs, 0, st>>>(n, xyz, g, rc2, (int32_t *)start.p, (int32_t *)atoms.p, nullptr,
+                                   (int64_t *)counts.p, nullptr);
+    HIP_TRY(hipGetLastError());
+    k_scan_i64<<<1, 1024, 0, st>>>(n, (int64_t *)counts.p, indptr);
+    HIP_TRY(hipGetLastError());
+    int64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nnz_out) *nnz_out = total;
+    return INCH_OK;
+  }
+  int64_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (total > capacity) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist_pbc: need %lld > capacity %lld", (long long)total, (long long)capacity);
+  k_neigh_pbc<<<gb, bs, 0, st>>>(n, xyz, g, rc2, (int32_t *)start.p, (int32_t *)atoms.p, indptr, nullptr, indices);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMalloc(&ovf.p, sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(ovf.p, 0, sizeof(int32_t), st));
+  k_sort_rows<<<n, 256, 0, st>>>(n, indptr, indices, (int32_t *)ovf.p);
+  HIP_TRY(hipGetLastError());
+  int32_t over = 0;
+  HIP_TRY(hipMemcpyAsync(&over, ovf.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (over) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist_pbc: a row has %d neighbours (> %d)", over, kMaxRow);
+  k_image_masks<<<gb, bs, 0, st>>>(n, xyz, box_x, box_y, rc2, indptr, indices, images);
+  HIP_TRY(hipGetLastError());
+  if (nnz_out) *nnz_out = total;
+  return INCH_OK;
+}
+
+extern "C" inch_status inch_hessian_assemble_pbc(int64_t n, const double *xyz, double plus_i, const int64_t *rowptr,
+                                                 const int32_t *col, const uint16_t *images, double box_x,
+                                                 double box_y, double *vals64, float *vals32, double *diag,
+                                                 void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && rowptr && col && images && box_x > 0 && box_y > 0, "inch_hessian_assemble_pbc: bad args");
+  INCH_REQUIRE(vals64 || vals32 || diag, "inch_hessian_assemble_pbc: no output");
+  hipStream_t st = (hipStream_t)stream;
+  const int bs = 256;
+  const int64_t grid = (n * 64 + bs - 1) / bs;
+  k_assemble<<<grid, bs, 0, st>>>(n, xyz, plus_i, rowptr, col, images, box_x, box_y, vals64, vals32, diag);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <vector>
 
 #include "common.h"
 
@@ -227,7 +228,8 @@ int64_t inch_rcm_device_work_size(int64_t n) { return 6 * (n + 2) * (int64_t)siz
 
 inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *indices, const int64_t *seeds,
                             int32_t reverse, int32_t *order, int64_t *undo, void *work, int64_t work_bytes,
-                            void *stream) {
+                            int64_t *comp_ptr, int64_t comp_cap, int64_t *n_comp, void *stream) {
+  INCH_REQUIRE(!comp_ptr || comp_cap >= 2, "inch_rcm_device: comp_cap must be >= 2");
   INCH_REQUIRE(n > 0 && n < INT_MAX - 2 && indptr && indices && seeds && order && undo && work,
                "inch_rcm_device: bad args");
   INCH_REQUIRE(work_bytes >= inch_rcm_device_work_size(n), "inch_rcm_device: work too small (%lld < %lld)",
@@ -262,8 +264,10 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
   k_rcm_init<<<blocks_for(n, bs), bs, 0, st>>>(n, pos, claim);
   HIP_TRY(hipGetLastError());
   int64_t N = 0, z = 0;
+  std::vector<int64_t> starts;   // BFS position of each component's seed
   while (N < n) {
     if (z >= n) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: seeds is not a permutation");
+    starts.push_back(N);
     k_rcm_best_init<<<1, 1, 0, st>>>(best);
     k_rcm_next_seed<<<blocks_for(n - z, bs), bs, 0, st>>>(n, z, seeds, pos, best);
     unsigned long long kbest = 0;
@@ -298,6 +302,17 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
   k_rcm_finish<<<blocks_for(n, bs), bs, 0, st>>>(n, ord, reverse, order, undo);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
+  // components are contiguous in the output order: their start offsets, ascending, then n
+  const int64_t C = (int64_t)starts.size();
+  if (n_comp) *n_comp = C;
+  if (comp_ptr) {
+    if (C + 1 > comp_cap) INCH_FAIL(INCH_ERR_CAPACITY, "inch_rcm_device: %lld components exceed comp_cap", (long long)C);
+    for (int64_t c = 0; c < C; ++c) {
+      const int64_t s0 = starts[c], s1 = c + 1 < C ? starts[c + 1] : n;
+      comp_ptr[reverse ? C - 1 - c : c] = reverse ? n - s1 : s0;
+    }
+    comp_ptr[C] = n;
+  }
   return INCH_OK;
 }
 

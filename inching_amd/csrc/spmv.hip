@@ -255,6 +255,75 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
   }
 }
 
+// Periodic images in x and y: the entry (i, j) carries a 9-bit mask of the images
+// s = (sx, sy) of atom j within the cutoff (bit (sx + 1) * 3 + (sy + 1)), and the product sums
+// d_s (d_s . (x_i - x_j)) / r_s^2 over them, d_s = (p_j + s box) - p_i. 16 lanes per row.
+// Several right-hand sides (row-major [3n, nrhs]) when nrhs > 1; the epilogue applies to
+// nrhs == 1 only.
+template <typename VT>
+__global__ void __launch_bounds__(256) k_matfree_pbc(int64_t n, const double *__restrict__ xyz,
+                                                      const int64_t *__restrict__ rowptr,
+                                                      const int32_t *__restrict__ col,
+                                                      const uint16_t *__restrict__ images, double bx, double by,
+                                                      double plus_i, const VT *__restrict__ x, VT *__restrict__ y,
+                                                      int nrhs, double s1, double c0, const VT *__restrict__ z,
+                                                      double c2, VT *__restrict__ acc, double mu,
+                                                      const double *__restrict__ Q, const double *__restrict__ qcoef,
+                                                      int mq) {
+  constexpr int G = 16;
+  const int lane = threadIdx.x & (G - 1);
+  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int r = blockIdx.y;
+  if (row >= n || r >= nrhs) return;
+  const double pi0 = xyz[3 * row], pi1 = xyz[3 * row + 1], pi2 = xyz[3 * row + 2];
+  const double vi0 = (double)x[(3 * row) * nrhs + r], vi1 = (double)x[(3 * row + 1) * nrhs + r],
+               vi2 = (double)x[(3 * row + 2) * nrhs + r];
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int64_t b = rowptr[row] + lane; b < rowptr[row + 1]; b += G) {
+    const int64_t j = col[b];
+    const uint32_t m = images[b];
+    const double w0 = vi0 - (double)x[(3 * j) * nrhs + r], w1 = vi1 - (double)x[(3 * j + 1) * nrhs + r],
+                 w2 = vi2 - (double)x[(3 * j + 2) * nrhs + r];
+    const double d2 = xyz[3 * j + 2] - pi2;
+    for (int bit = 0; bit < 9; ++bit) {
+      if (!((m >> bit) & 1u)) continue;
+      const double d0 = (xyz[3 * j] + (double)(bit / 3 - 1) * bx) - pi0;
+      const double d1 = (xyz[3 * j + 1] + (double)(bit % 3 - 1) * by) - pi1;
+      const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      const double t = r2 > 0.0 ? (d0 * w0 + d1 * w1 + d2 * w2) * recip_r2(r2) : 0.0;
+      a0 += d0 * t;
+      a1 += d1 * t;
+      a2 += d2 * t;
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1) {
+    a0 += __shfl_xor(a0, off, G);
+    a1 += __shfl_xor(a1, off, G);
+    a2 += __shfl_xor(a2, off, G);
+  }
+  if (lane < 3) {
+    const int64_t dof = 3 * row + lane;
+    const double vi = lane == 0 ? vi0 : (lane == 1 ? vi1 : vi2);
+    double out = (lane == 0 ? a0 : (lane == 1 ? a1 : a2)) + plus_i * vi;
+    if (nrhs > 1) {
+      y[dof * nrhs + r] = (VT)out;
+      return;
+    }
+    if (c0 != 0.0) out -= c0 * vi;
+    out *= s1;
+    if (z) out -= c2 * (double)z[dof];
+    if (qcoef) {
+      double q = 0.0;
+      const int64_t nd = 3 * n;
+      for (int mm = 0; mm < mq; ++mm) q += qcoef[mm] * Q[mm * nd + dof];
+      out += q;
+    }
+    y[dof] = (VT)out;
+    if (acc) acc[dof] = (VT)((double)acc[dof] + mu * out);
+  }
+}
+
 // Streaming copy used to measure the achievable HBM bandwidth (16 B per lane access).
 // One 16-B element per lane and one pass over the grid (no grid-stride loop): on MI355X this
 // form streams at ~6.2 TB/s, against ~4.6 TB/s for a grid-stride loop of the same accesses.
@@ -565,6 +634,33 @@ inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *r
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: bad dtype");
   if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: config %d is not lanes*10+unroll with lanes in "
                                        "{4,8,16,32} and unroll in {1,2}", code);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_anm_matfree_pbc(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                 const uint16_t *images, double box_x, double box_y, double plus_i, int32_t vdt,
+                                 const void *x, void *y, int32_t nrhs, double s1, double c0, const void *z,
+                                 double c2, void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
+                                 void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && rowptr && col && images && x && y && x != y && nrhs > 0 && nrhs <= 65535,
+               "inch_anm_matfree_pbc: bad args");
+  INCH_REQUIRE(nrhs == 1 || (!z && !acc && !qcoef && s1 == 1.0 && c0 == 0.0),
+               "inch_anm_matfree_pbc: the epilogue needs nrhs == 1");
+  INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_pbc: qcoef needs Q");
+  hipStream_t st = (hipStream_t)stream;
+  const int bs = 256;
+  dim3 grid((unsigned)((n * 16 + bs - 1) / bs), (unsigned)nrhs);
+  if (vdt == INCH_F64)
+    k_matfree_pbc<double><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, images, box_x, box_y, plus_i, (const double *)x,
+                                               (double *)y, nrhs, s1, c0, (const double *)z, c2, (double *)acc, mu, Q,
+                                               qcoef, m_q);
+  else if (vdt == INCH_F32)
+    k_matfree_pbc<float><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, images, box_x, box_y, plus_i, (const float *)x,
+                                              (float *)y, nrhs, s1, c0, (const float *)z, c2, (float *)acc, mu, Q,
+                                              qcoef, m_q);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_pbc: bad dtype");
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

@@ -29,7 +29,10 @@ class BlockHessian:
     ``Burn/Krylov/T3.py:131-160``. ``A.matvec(v)`` is the fused full product.
     """
 
-    def __init__(self, xyz, rowptr, col, vals64, vals32, diag, plus_i, rc, order=None, undo=None):
+    def __init__(self, xyz, rowptr, col, vals64, vals32, diag, plus_i, rc, order=None, undo=None, images=None,
+                 box=None):
+        self.images = images        # periodic-image masks per entry (None: open boundaries)
+        self.box = box              # (box_x, box_y) of the periodic images
         self.xyz = xyz
         self.rowptr = rowptr
         self.col = col
@@ -68,8 +71,13 @@ class BlockHessian:
     def ensure(self, precision):
         """Make sure the values of ``precision`` exist (assembled on demand)."""
         if (precision == "f64" and self.vals64 is None) or (precision == "f32" and self.vals32 is None):
-            v64, v32, _ = ops.hessian_assemble(self.xyz, self.rc, self.plus_i, self.rowptr, self.col,
-                                               want64=precision == "f64", want32=precision == "f32")
+            if self.images is not None:
+                v64, v32, _ = ops.hessian_assemble_pbc(self.xyz, self.plus_i, self.rowptr, self.col, self.images,
+                                                       self.box, want64=precision == "f64",
+                                                       want32=precision == "f32")
+            else:
+                v64, v32, _ = ops.hessian_assemble(self.xyz, self.rc, self.plus_i, self.rowptr, self.col,
+                                                   want64=precision == "f64", want32=precision == "f32")
             if precision == "f64":
                 self.vals64 = v64
             else:
@@ -94,13 +102,19 @@ class BlockHessian:
 
     def matfree_spmv(self, x, y, config=0, **epi):
         """Matrix-free float64 product with the fused epilogue (``config``: autotuner code
-        lanes*10+unroll, 0 = default)."""
-        ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config, **epi)
+        lanes*10+unroll, 0 = default; periodic matrices use their own kernel)."""
+        if self.images is not None:
+            ops.matfree_pbc(self.xyz, self.rowptr, self.col, self.images, self.box, self.plus_i, x, y, **epi)
+        else:
+            ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config, **epi)
 
     def matfree(self, X):
         """FP64 (H + PlusI I) X recomputed from coordinates (X [3n] or [3n, r])."""
         Y = torch.empty_like(X)
-        ops.matfree_apply(self.xyz, self.rowptr, self.col, self.plus_i, X.contiguous(), Y)
+        if self.images is not None:
+            ops.matfree_pbc(self.xyz, self.rowptr, self.col, self.images, self.box, self.plus_i, X.contiguous(), Y)
+        else:
+            ops.matfree_apply(self.xyz, self.rowptr, self.col, self.plus_i, X.contiguous(), Y)
         return Y
 
     def _part(self, v, part):
@@ -168,21 +182,31 @@ def rcm_order(X, rc, seed_order=None, reverse=True, device=None, rcm="device"):
     return ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order, reverse=reverse)
 
 
-def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_order=None, device=None, rcm="device"):
+def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_order=None, device=None, rcm="device",
+                  box=None, origin=(0.0, 0.0)):
     """Coordinates -> BlockHessian (in RCM order when ``reorder``).
 
     ``precision``: which value copies to assemble ('f32', 'f64'). Returns the container;
     ``A.order`` / ``A.undo`` hold the permutation (identity when not reordered). ``rcm``:
-    'device' or 'host' ordering (the same permutation).
+    'device' or 'host' ordering (the same permutation). ``box`` = (box_x, box_y): periodic
+    images in x and y (box corner ``origin``); the ordering then runs on the periodic
+    neighbour graph.
     """
     ops.require_gpu()
     device = device or torch.device("cuda", torch.cuda.current_device())
     xyz = _to_device_xyz(X, device)
     n = xyz.shape[0]
+
+    def pattern(xyz_):
+        if box is None:
+            return ops.neighbours(xyz_, rc) + (None,)
+        return ops.neighbours_pbc(xyz_, rc, box, origin)
+
+    comps = None                     # connected components (offsets in the RCM order)
     if reorder:
-        indptr, indices = ops.neighbours(xyz, rc)
+        indptr, indices, _ = pattern(xyz)
         if rcm == "device":
-            order_d, undo_d = ops.rcm_device(indptr, indices, seed_order=seed_order)
+            order_d, undo_d, comps = ops.rcm_device(indptr, indices, seed_order=seed_order, components=True)
             xyz = xyz[order_d.long()].contiguous()
             order, undo = order_d.cpu().numpy(), undo_d.cpu().numpy()
         else:
@@ -192,17 +216,25 @@ def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_orde
     else:
         order = np.arange(n, dtype=np.int32)
         undo = np.arange(n, dtype=np.int64)
-    rowptr, col = ops.neighbours(xyz, rc)
-    v64, v32, diag = ops.hessian_assemble(xyz, rc, plus_i, rowptr, col, want64="f64" in precision,
-                                          want32="f32" in precision)
-    return BlockHessian(xyz, rowptr, col, v64, v32, diag, plus_i, rc, order=order, undo=undo)
+    rowptr, col, images = pattern(xyz)
+    if images is None:
+        v64, v32, diag = ops.hessian_assemble(xyz, rc, plus_i, rowptr, col, want64="f64" in precision,
+                                              want32="f32" in precision)
+    else:
+        v64, v32, diag = ops.hessian_assemble_pbc(xyz, plus_i, rowptr, col, images, box, want64="f64" in precision,
+                                                  want32="f32" in precision)
+    A = BlockHessian(xyz, rowptr, col, v64, v32, diag, plus_i, rc, order=order, undo=undo, images=images,
+                     box=None if box is None else (float(box[0]), float(box[1])))
+    A.components = comps
+    return A
 
 
-def rigid_body_basis(xyz):
+def rigid_body_basis(xyz, rotations=True):
     """6 x 3N rigid-body basis of device coordinates, built as the reference driver builds it
     (Command8A/commandBenchmark_InchingTRLMHD_Linux_Pdb_0064_A100.py:314-372): three unit
     translations (1/sqrt N), three infinitesimal rotations X @ R_a normalised, then two sweeps
-    of double Gram-Schmidt. float64 torch [6, 3N] on the device of ``xyz``."""
+    of double Gram-Schmidt. float64 torch [6, 3N] on the device of ``xyz``. With periodic
+    images only the translations are zero modes (``rotations=False``: the 3 x 3N block)."""
     x = xyz.to(torch.float64)
     n = x.shape[0]
     Q = torch.zeros((6, 3 * n), dtype=torch.float64, device=x.device)
@@ -220,7 +252,7 @@ def rigid_body_basis(xyz):
             for _rep in range(2):
                 Q[ix] -= Q[:ix].T @ (Q[:ix] @ Q[ix])
                 Q[ix] /= torch.sqrt(Q[ix] @ Q[ix])
-    return Q
+    return Q if rotations else Q[:3].contiguous()
 
 
 def component_rigid_basis(xyz, comp):

@@ -72,11 +72,12 @@ def rcm_host(indptr, indices, seed_order=None, reverse=True):
     return order, undo
 
 
-def rcm_device(indptr, indices, seed_order=None, reverse=True, stream=None):
+def rcm_device(indptr, indices, seed_order=None, reverse=True, stream=None, components=False):
     """3DRCM on the device: (order int32 [n], undo int64 [n]) device tensors, bit-exact to
     rcm_host. indptr int64 / indices int32 device tensors (unique columns per row).
     seed_order: optional seed order (default: stable argsort of the degree, computed on the
-    host from the row pointers with a radix sort)."""
+    host from the row pointers with a radix sort). ``components``: also return the start
+    offsets of the connected components in the output order (numpy int64 [C + 1])."""
     n = indptr.numel() - 1
     dev = indptr.device
     if n <= 0:
@@ -93,8 +94,13 @@ def rcm_device(indptr, indices, seed_order=None, reverse=True, stream=None):
     undo = torch.empty(n, dtype=torch.int64, device=dev)
     wb = int(call("inch_rcm_device_work_size", n))
     work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    cap = n + 1
+    comp = np.zeros(cap, dtype=np.int64) if components else None
+    ncomp = C.c_int64(0)
     call("inch_rcm_device", n, P(indptr), P(indices), P(seeds), 1 if reverse else 0, P(order), P(undo), P(work), wb,
-         stream_ptr(stream))
+         comp.ctypes.data_as(C.c_void_p) if components else None, cap, C.byref(ncomp), stream_ptr(stream))
+    if components:
+        return order, undo, comp[:ncomp.value + 1].copy()
     return order, undo
 
 
@@ -122,6 +128,65 @@ def neighbours(xyz, rc, stream=None):
     call("inch_neighbors_celllist", n, P(xyz), float(rc), P(indptr), P(indices), nnz.value, C.byref(nnz),
          stream_ptr(stream))
     return indptr, indices[:nnz.value]
+
+
+def neighbours_pbc(xyz, rc, box, origin, stream=None):
+    """Neighbour CSR with periodic images in x and y: (indptr, indices, images), images[e]
+    the 9-bit mask of the images of column j within the cutoff (see inch_neighbors_celllist_pbc).
+    box = (box_x, box_y), origin = (x0, y0) of the periodic box."""
+    require_gpu()
+    _check_dev(xyz)
+    if xyz.dtype != torch.float64 or xyz.dim() != 2 or xyz.shape[1] != 3:
+        raise ValueError("xyz must be float64 [n,3]")
+    n = xyz.shape[0]
+    bx, by = float(box[0]), float(box[1])
+    ox, oy = float(origin[0]), float(origin[1])
+    indptr = torch.empty(n + 1, dtype=torch.int64, device=xyz.device)
+    nnz = C.c_int64(0)
+    call("inch_neighbors_celllist_pbc", n, P(xyz), float(rc), bx, by, ox, oy, P(indptr), None, None, 0,
+         C.byref(nnz), stream_ptr(stream))
+    indices = torch.empty(max(1, nnz.value), dtype=torch.int32, device=xyz.device)
+    images = torch.empty(max(1, nnz.value), dtype=torch.uint16, device=xyz.device)
+    call("inch_neighbors_celllist_pbc", n, P(xyz), float(rc), bx, by, ox, oy, P(indptr), P(indices), P(images),
+         nnz.value, C.byref(nnz), stream_ptr(stream))
+    return indptr, indices[:nnz.value], images[:nnz.value]
+
+
+def hessian_assemble_pbc(xyz, plus_i, rowptr, col, images, box, want64=True, want32=True, stream=None):
+    require_gpu()
+    _check_dev(xyz, rowptr, col)
+    n = xyz.shape[0]
+    nnzb = col.shape[0]
+    if rowptr.shape[0] != n + 1 or images.shape[0] != nnzb:
+        raise ValueError("rowptr / images inconsistent with the pattern")
+    v64 = torch.empty(nnzb * 9, dtype=torch.float64, device=xyz.device) if want64 else None
+    v32 = torch.empty(nnzb * 9, dtype=torch.float32, device=xyz.device) if want32 else None
+    diag = torch.empty(3 * n, dtype=torch.float64, device=xyz.device)
+    call("inch_hessian_assemble_pbc", n, P(xyz), float(plus_i), P(rowptr), P(col), P(images), float(box[0]),
+         float(box[1]), P(v64), P(v32), P(diag), stream_ptr(stream))
+    return v64, v32, diag
+
+
+def matfree_pbc(xyz, rowptr, col, images, box, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0,
+                Q=None, qcoef=None, stream=None):
+    """Matrix-free ANM product with periodic images; x, y [3n] (fused epilogue) or [3n, nrhs]."""
+    n = xyz.shape[0]
+    nd = 3 * n
+    nrhs = 1 if x.dim() == 1 else x.shape[1]
+    if x.shape[0] != nd or y.shape != x.shape:
+        raise ValueError("x, y must be [3n] or [3n, nrhs]")
+    vd = dt(x)
+    for t in (y, z, acc):
+        if t is not None and dt(t) != vd:
+            raise ValueError("all vectors must share a dtype")
+    mq = 0
+    if qcoef is not None:
+        if Q is None or Q.dtype != torch.float64 or Q.shape[1] != nd:
+            raise ValueError("Q must be float64 [m, 3n]")
+        mq = Q.shape[0]
+    call("inch_anm_matfree_pbc", n, P(xyz), P(rowptr), P(col), P(images), float(box[0]), float(box[1]),
+         float(plus_i), vd, P(x), P(y), nrhs, float(s1), float(c0), P(z), float(c2), P(acc), float(mu), P(Q),
+         P(qcoef), mq, stream_ptr(stream))
 
 
 def hessian_assemble(xyz, rc, plus_i, rowptr, col, want64=True, want32=True, stream=None):

@@ -84,11 +84,13 @@ inch_status inch_rcm_host(int64_t n, const int64_t *indptr, const int32_t *indic
  * indices (the neighbour CSR). seeds: int64[n] seed order, a permutation (the stable
  * argsort of the degree reproduces inch_rcm_host's default). work: device scratch of
  * inch_rcm_device_work_size(n) bytes. Syncs the stream (one level size per BFS level
- * comes back to the host). Inputs are validated on the device before use. */
+ * comes back to the host). Inputs are validated on the device before use. Connected
+ * components are contiguous in the order; when comp_ptr (host, comp_cap entries) is given it
+ * receives their start offsets followed by n, and *n_comp (host, optional) their number. */
 int64_t inch_rcm_device_work_size(int64_t n);
 inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *indices, const int64_t *seeds,
                             int32_t reverse, int32_t *order, int64_t *undo, void *work, int64_t work_bytes,
-                            void *stream);
+                            int64_t *comp_ptr, int64_t comp_cap, int64_t *n_comp, void *stream);
 
 /* Row-batch heads of recursive halving with leaves <= maxleaf, sorted descending
  * and cumulated: head[0..nbatch]. cap = capacity of head. Host only. */
@@ -153,6 +155,29 @@ inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *r
                                   double plus_i, int32_t vdt, const void *x, void *y, double s1, double c0,
                                   const void *z, double c2, void *acc, double mu, const double *Q,
                                   const double *qcoef, int32_t m_q, int32_t config, void *stream);
+
+/* Periodic images in x and y (membrane boxes; replaces X_cKDTreePbcXy,
+ * Fuel/Coordinate/T1.py:67-161, and the image terms of the Hessian builder,
+ * Fuel/T1.py:98-128,317-396). Image s = (sx, sy) in {-1,0,1}^2 shifts atom j by
+ * (sx*box_x, sy*box_y, 0); d_s = (x_j + s box) - x_i. An entry (i, j) of the neighbour CSR
+ * exists when any image is within the cutoff; images[e] (uint16) holds the mask of those
+ * images, bit (sx+1)*3 + (sy+1). box_x, box_y > rc (else "ABORTED." as the reference);
+ * (origin_x, origin_y) is the box corner used to wrap coordinates into periodic cells.
+ * Two-pass calling convention as inch_neighbors_celllist (images NULL in the count pass). */
+inch_status inch_neighbors_celllist_pbc(int64_t n, const double *xyz, double rc, double box_x, double box_y,
+                                        double origin_x, double origin_y, int64_t *indptr, int32_t *indices,
+                                        uint16_t *images, int64_t capacity, int64_t *nnz_out, void *stream);
+/* H_ij = sum over the images in the mask of -d_s d_s^T / |d_s|^2; diagonal as the open case. */
+inch_status inch_hessian_assemble_pbc(int64_t n, const double *xyz, double plus_i, const int64_t *rowptr,
+                                      const int32_t *col, const uint16_t *images, double box_x, double box_y,
+                                      double *vals64, float *vals32, double *diag, void *stream);
+/* Matrix-free product with images: y = (H + plus_i I) x for nrhs row-major right-hand sides
+ * (x, y [3n, nrhs]); with nrhs == 1 the epilogue of inch_anm_matfree_spmv applies. */
+inch_status inch_anm_matfree_pbc(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                 const uint16_t *images, double box_x, double box_y, double plus_i, int32_t vdt,
+                                 const void *x, void *y, int32_t nrhs, double s1, double c0, const void *z,
+                                 double c2, void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
+                                 void *stream);
 
 /* dst = src (bytes, multiple of 16): streaming copy for measuring achievable HBM bandwidth. */
 inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *stream);

@@ -31,6 +31,11 @@ hessian.*                     closed form -d d^T/|d|^2; invariants (symmetry,
                               null space of H, tr H = 2 #contacts, PSD);
                               dense <-> CSR agreement.
 hessian.rigid_body_basis      orthonormality Q Q^T = I_6 and H Q^T = 0.
+hessian.block_hessian_pbc     periodic x/y images by brute force over all pairs
+                              and nine shifts: symmetry, translations in the
+                              null space, exactly three zero modes, summed
+                              multi-image entries, equal to block_hessian
+                              when the box is far larger than the structure.
 chebyshev.damping_kernel      closed form (Jackson / Lanczos sigma).
 chebyshev.optimize_on_degree  golden vectors written by importing the
                               reference's numpy-only PolynomialFilters/T0.py

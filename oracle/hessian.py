@@ -119,3 +119,51 @@ def relative_residual(A_apply, lam, V, lam_max):
     """||A v - lam v||_2 / (lam_max ||v||_2) per column (the build's rel_res definition)."""
     R = A_apply(V) - V * lam[None, :]
     return np.linalg.norm(R, axis=0) / (lam_max * np.linalg.norm(V, axis=0))
+
+
+def block_hessian_pbc(X, rc, plus_i, box):
+    """Periodic x/y images (``Fuel/Coordinate/T1.py:67-161``, ``Fuel/T1.py:98-128,317-396``),
+    brute force over all pairs and the nine images s = (sx, sy) in {-1,0,1}^2:
+    d_s = (x_j + s * box) - x_i in float64, j a neighbour of i when some image has
+    |d_s|^2 <= rc^2 (self included), H_ij = sum over those images of -d_s d_s^T / |d_s|^2
+    (|d_s|^2 > 0), H_ii = -sum_{j != i} H_ij + plus_i I. Returns (rowptr, col, vals [nnzb,3,3],
+    image masks, bit (sx+1)*3 + (sy+1))."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[0]
+    rc2 = float(rc) * float(rc)
+    shifts = [(sx, sy) for sx in (-1, 0, 1) for sy in (-1, 0, 1)]
+    rows, cols, blocks, masks = [], [], [], []
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    for i in range(n):
+        dz = X[:, 2] - X[i, 2]
+        m = np.zeros(n, dtype=np.int64)
+        dd = {}
+        for bit, (sx, sy) in enumerate(shifts):
+            dx = (X[:, 0] + sx * box[0]) - X[i, 0]
+            dy = (X[:, 1] + sy * box[1]) - X[i, 1]
+            d2 = (dx * dx + dy * dy) + dz * dz
+            hit = d2 <= rc2
+            m[hit] |= 1 << bit
+            dd[bit] = (dx, dy, d2)
+        js = np.nonzero(m)[0]
+        diag = np.zeros((3, 3))
+        self_pos = None
+        for j in js:
+            if j == i:
+                self_pos = len(blocks)
+                blocks.append(None)
+            else:
+                h = np.zeros((3, 3))
+                for bit in range(9):
+                    if (m[j] >> bit) & 1:
+                        dx, dy, d2 = dd[bit]
+                        if d2[j] > 0:
+                            d = np.array([dx[j], dy[j], dz[j]])
+                            h -= np.outer(d, d) / d2[j]
+                blocks.append(h)
+                diag -= h
+            cols.append(j)
+            masks.append(int(m[j]))
+        blocks[self_pos] = diag + plus_i * np.eye(3)
+        rowptr[i + 1] = len(cols)
+    return rowptr, np.asarray(cols, dtype=np.int32), np.asarray(blocks), np.asarray(masks, dtype=np.int64)

@@ -89,8 +89,17 @@ def test_rcm_device_bit_exact(glob2k, glob40k, case):
     ip, ix = ip_d.cpu().numpy(), ix_d.cpu().numpy()
     seeds = np.argsort(np.diff(ip)) if case == "numpy_default_seeds" else None
     reverse = case != "forward"
-    o, u = ops.rcm_device(ip_d, ix_d, seed_order=seeds, reverse=reverse)
+    o, u, comps = ops.rcm_device(ip_d, ix_d, seed_order=seeds, reverse=reverse, components=True)
     o, u = o.cpu().numpy(), u.cpu().numpy()
+    # components are contiguous in the order and their offsets match scipy's labelling
+    from scipy.sparse.csgraph import connected_components
+    nc, lab = connected_components(sp.csr_matrix((np.ones(ix.size), ix, ip), shape=(len(ip) - 1,) * 2),
+                                   directed=False)
+    lab_o = lab[o]
+    want = np.r_[np.nonzero(np.r_[True, lab_o[1:] != lab_o[:-1]])[0], len(o)]
+    assert len(want) == nc + 1 and np.array_equal(comps, want)
+    if case == "blobs_and_singletons":
+        assert nc == 5
     if case == "40k":
         o_ref, u_ref = ops.rcm_host(ip, ix)
     else:

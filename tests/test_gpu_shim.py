@@ -174,3 +174,25 @@ def test_irlm_shim_matches_dense(workflow):
     lam, V = S_HeigvalIRLMHD_HeigvecIRLMHD(A, k=k, tol=1e-10, User_Q_HotellingDeflation=workflow["Q"],
                                            User_HotellingShift=10.0, maxiter=3000)
     assert np.allclose(lam.cpu().numpy(), w[6:6 + k], atol=1e-7)
+
+
+def test_cdsi_shim_matches_dense(workflow):
+    """Chebyshev-Davidson (no deflation, as the reference): the lowest 16 eigenpairs including
+    the six rigid-body modes at PlusI; the assertions of the reference."""
+    from InchingLiteInteger.Burn.ChebyshevDavidsonSubspaceIteration.T1 import S_HeigvalCDSIHD_HeigvecCDSIHD
+    from InchingLiteInteger.Burn.HermitianLanczos.T2 import A_Adiag_EstimateSpectrumBound
+    print("case: shim CDSI k=16", flush=True)
+    A, A_diag, w = workflow["A"], workflow["A_diag"], workflow["w"]
+    bound = A_Adiag_EstimateSpectrumBound(A, A_diag)
+    bound = (1.0, bound[1].get() + 1e-12)
+    k = 16
+    lam, V = S_HeigvalCDSIHD_HeigvecCDSIHD(A, A_diag, k=k, tol=1e-12, maxiter=20000, User_ChebyshevDegree=20,
+                                           User_SpectrumBound=bound)
+    lam = lam.cpu().numpy()
+    assert np.allclose(lam, w[:k], atol=1e-9)
+    assert _driver_residuals(A, A_diag, lam, V).max() < 1e-8
+    with pytest.raises(AssertionError, match="ABORTED"):
+        S_HeigvalCDSIHD_HeigvecCDSIHD(A, A_diag, k=k, User_SpectrumBound=None)
+    with pytest.raises(AssertionError, match="ABORTED"):
+        S_HeigvalCDSIHD_HeigvecCDSIHD(A, A_diag, k=k, User_SpectrumBound=bound,
+                                       User_Q_HotellingDeflation=workflow["Q"])

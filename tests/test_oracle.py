@@ -249,6 +249,33 @@ def test_hessian_invariants():
     assert w[0] > -1e-10 and np.sum(np.abs(w) < 1e-9) == 6
 
 
+def test_periodic_hessian_invariants():
+    """Periodic x/y images: symmetric, translations in the null space, exactly three zero
+    modes (rotations are no longer free), multi-image entries summed when the box is below
+    twice the cutoff; with a box far larger than the structure it equals the open Hessian."""
+    rng = np.random.default_rng(8)
+    X = rng.uniform(0, 15, (70, 3))
+    X[:, 2] *= 0.5
+    rp, col, v, m = OH.block_hessian_pbc(X, 8.0, 0.0, (15.0, 15.0))
+    n = 70
+    D = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for e in range(rp[i], rp[i + 1]):
+            D[3 * i:3 * i + 3, 3 * col[e]:3 * col[e] + 3] = v[e]
+    assert np.abs(D - D.T).max() <= 1e-13
+    for a in range(3):
+        t = np.zeros(3 * n)
+        t[a::3] = 1.0
+        assert np.abs(D @ t).max() <= 1e-12
+    w = np.linalg.eigvalsh(D)
+    assert np.sum(np.abs(w) < 1e-9) == 3 and w.min() > -1e-9
+    assert any(bin(int(x)).count("1") > 1 for x in m)
+    rp2, col2, v2, m2 = OH.block_hessian_pbc(X, 8.0, 1.0, (1000.0, 1000.0))
+    rp3, col3, v3 = OH.block_hessian(X, 8.0, 1.0)
+    assert np.array_equal(rp2, rp3) and np.array_equal(col2, col3) and np.all(m2 == 1 << 4)
+    assert np.max(np.abs(v2 - v3.reshape(v2.shape))) <= 1e-14
+
+
 def test_lower_csr_and_halfmem_matvec():
     X = small_globule(300, seed=6)
     L, d = OH.lower_csr_and_diag(X, 8.0, plus_i=1.0)
