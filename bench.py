@@ -66,14 +66,15 @@ def spmv_only(args):
     z32, y32, acc32 = torch.randn_like(x32), torch.empty_like(x32), torch.zeros_like(x32)
     x64 = torch.randn(n, dtype=torch.float64, device="cuda")
     z64, y64, acc64 = torch.randn_like(x64), torch.empty_like(x64), torch.zeros_like(x64)
-    from inching_amd.autotune import best_config
+    from inching_amd.autotune import best_config, bsr_launch
     cfg = {p: best_config(A, p) for p in ("f32", "f64", "mf64")}       # the launch shapes the solver uses
+    l32, l64 = bsr_launch(A, cfg["f32"]), bsr_launch(A, cfg["f64"])
     out["kernels"] = {p: kernel_key(p, A) for p in cfg}
     cases = {
         "bsr_f32_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x32, y32, s1=0.1, c0=2.0, z=z32,
-                                             c2=1.0, acc=acc32, mu=0.01, lanes=cfg["f32"]),
+                                             c2=1.0, acc=acc32, mu=0.01, lanes=l32[0], col16=l32[1]),
         "bsr_f64_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals64, x64, y64, s1=0.1, c0=2.0, z=z64,
-                                             c2=1.0, acc=acc64, mu=0.01, lanes=cfg["f64"]),
+                                             c2=1.0, acc=acc64, mu=0.01, lanes=l64[0], col16=l64[1]),
         "matfree_f64_cheb": lambda: A.matfree_spmv(x64, y64, s1=0.1, c0=2.0, z=z64, c2=1.0, acc=acc64, mu=0.01,
                                                    config=cfg["mf64"]),
     }
@@ -274,7 +275,9 @@ def kernel_key(kind, A):
     if kind == "mf64":
         return "k_matfree_epi<double,%d,%d>" % (cfg // 10, cfg % 10)
     t = "float" if kind == "f32" else "double"
-    return "k_bsr_spmv<%s,%s,%d>" % (t, t, cfg)
+    if cfg >= 200:
+        return "k_bsr_spmv<%s,%s,%d,unsigned short>" % (t, t, cfg - 200)
+    return "k_bsr_spmv<%s,%s,%d,int>" % (t, t, cfg)
 
 
 def measured_traffic(key):
@@ -295,8 +298,8 @@ def measured_traffic(key):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; --gpus %d is not supported (single-GPU path, "
-              "see DESIGN.md for how the solve would shard)" % args.gpus, file=sys.stderr)
+        print("bench.py: measures one GPU; --gpus %d is not supported (the solve runs on a single GPU)"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.spmv_only:
         return spmv_only(args)

@@ -87,14 +87,29 @@ def best_config(A, precision, reps=5):
             times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
                                reps)
     else:
+        # int32 columns: code = lanes; 16-bit column offsets: code = 200 + lanes
         vals = A.vals(precision)
         for lanes in BSR_LANES:
             times[lanes] = _time(lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, s1=0.1, c0=1.0, z=z,
                                                       c2=1.0, acc=acc, mu=0.1, lanes=lanes), reps)
+        c16 = A.col16()
+        if c16 is not None:
+            for lanes in BSR_LANES:
+                times[200 + lanes] = _time(lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, s1=0.1,
+                                                                c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, lanes=lanes,
+                                                                col16=c16), reps)
     best = min(times, key=times.get)
     _cache[k] = {"best": int(best), "ms": {str(c): round(t, 5) for c, t in times.items()}}
     _save()
     return int(best)
+
+
+def bsr_launch(A, code):
+    """(lanes, col16 or None) of a block-SpMV config code."""
+    code = int(code)
+    if code >= 200:
+        return code - 200, A.col16()
+    return code, None
 
 
 def table():

@@ -333,14 +333,15 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
 
 
 def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, per_component=None,
-                   filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False):
+                   filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False, factor=3, retries=2):
     """Lowest ``k_total`` non-trivial modes of a structure made of disconnected components
     (e.g. subunits of an assembly farther apart than the cutoff). H is block-diagonal, so its
     spectrum is the union of the components' spectra: every component is solved for
-    ``per_component`` modes in one lockstep run (each with its own six rigid-body modes
-    deflated), the results are merged, and the merge is checked: a component whose
-    highest computed mode lies below the k_total-th merged eigenvalue could hide more modes
-    below it, which raises (increase ``per_component``)."""
+    ``per_component`` modes (default ceil(1.15 k_total / C) + 4) in one lockstep run (each with
+    its own six rigid-body modes deflated), the results are merged, and the merge is checked:
+    a component whose highest computed mode lies below the k_total-th merged eigenvalue could
+    hide more modes below it. Then the solve is repeated with twice the modes per component
+    (at most ``retries`` times), after which it raises."""
     from . import chebyshev as CH
     ops.require_gpu()
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -351,24 +352,28 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
     seg_atoms = components(A)
     C = len(seg_atoms) - 1
     t_build = time.time() - t0
-    ks = int(per_component) if per_component else int(np.ceil(1.4 * k_total / C)) + 4
-    pol = None
-    if filtered:
-        lo, hi = CH.spectrum_bound(A, precision=precision)
-        bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
-        pol, cnt, interval, _ = CH.optimize_on_memory(A, bound, A.plus_i, wanted_count=int(C * (ks + 6) * 1.5),
-                                                      designated_start=bound[1] / 10.0, max_degree=max_degree,
-                                                      min_degree=5, margin=max(5, C * ks // 10), trials=5,
-                                                      ratio=0.7, seed=seed, precision=precision)
-    r = solve_segments(A, seg_atoms, ks, precision=precision, tol_rel=tol_rel, maxiter=maxiter, seed=seed,
-                       verbose=verbose, t0=t0, pol=pol)
-    flat = r.eigval.reshape(-1)
-    order = np.argsort(flat, kind="stable")[:k_total]
-    cutoff = flat[order[-1]]
-    short = [s for s in range(C) if r.eigval[s, -1] <= cutoff]
-    if short:
-        raise RuntimeError("components %s may have more than %d modes below the %d-th eigenvalue; "
-                           "raise per_component" % (short[:10], ks, k_total))
+    ks = int(per_component) if per_component else int(np.ceil(1.15 * k_total / C)) + 4
+    lo, hi = CH.spectrum_bound(A, precision=precision) if filtered else (None, None)
+    for attempt in range(retries + 1):
+        pol = None
+        if filtered:
+            bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
+            pol, cnt, interval, _ = CH.optimize_on_memory(A, bound, A.plus_i, wanted_count=int(C * (ks + 6) * 1.5),
+                                                          designated_start=bound[1] / 10.0, max_degree=max_degree,
+                                                          min_degree=5, margin=max(5, C * ks // 10), trials=5,
+                                                          ratio=0.7, seed=seed, precision=precision)
+        r = solve_segments(A, seg_atoms, ks, precision=precision, tol_rel=tol_rel, maxiter=maxiter, seed=seed,
+                           factor=factor, verbose=verbose, t0=t0, pol=pol)
+        flat = r.eigval.reshape(-1)
+        order = np.argsort(flat, kind="stable")[:k_total]
+        cutoff = flat[order[-1]]
+        short = [s for s in range(C) if r.eigval[s, -1] <= cutoff]
+        if not short:
+            break
+        if attempt == retries:
+            raise RuntimeError("components %s may have more than %d modes below the %d-th eigenvalue; "
+                               "raise per_component" % (short[:10], ks, k_total))
+        ks = 2 * ks
     comp_of = order // ks
     mode_of = order % ks
     torch.cuda.synchronize()

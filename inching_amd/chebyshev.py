@@ -342,14 +342,15 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
     vals = A.vals(precision) if precision != "mf64" else None
     row = v.view(1, -1)
 
-    from .autotune import best_config
+    from .autotune import best_config, bsr_launch
     cfg = best_config(A, precision)
+    blanes, c16 = bsr_launch(A, cfg) if precision != "mf64" else (0, None)
 
     def av(x, y):
         if vals is None:
             A.matfree_spmv(x, y, config=cfg)
         else:
-            ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=cfg)
+            ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=blanes, col16=c16)
 
     av(v, f)
     ops.multidot(row, 1, f, alpha[0:1], ws)

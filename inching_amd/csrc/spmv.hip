@@ -23,9 +23,13 @@ namespace {
 
 constexpr int kRedBlocks = 512;  // fixed grid of the two-phase reductions (determinism)
 
-template <typename MT, typename VT, int G>
+// Column indices: absolute int32 (CT = int32_t, rowbase = nullptr), or 16-bit offsets from
+// the row's first column (CT = uint16_t, rowbase[row] = that column), 2 B per block instead
+// of 4 B when every row of the RCM band spans fewer than 65,536 columns.
+template <typename MT, typename VT, int G, typename CT>
 __global__ void __launch_bounds__(256) k_bsr_spmv(int64_t n, const int64_t *__restrict__ rowptr,
-                                                   const int32_t *__restrict__ col, const MT *__restrict__ vals,
+                                                   const CT *__restrict__ col, const int32_t *__restrict__ rowbase,
+                                                   const MT *__restrict__ vals,
                                                    const VT *__restrict__ x, VT *__restrict__ y, MT s1, MT c0,
                                                    const VT *__restrict__ z, MT c2, VT *__restrict__ acc, MT mu,
                                                    const double *__restrict__ Q, const double *__restrict__ qcoef,
@@ -34,10 +38,11 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int64_t n, const int64_t *__re
   const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
   if (row >= n) return;  // whole groups leave together (a group never spans two rows)
   const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
+  const int64_t cb = rowbase ? (int64_t)rowbase[row] : 0;
   MT a0 = 0, a1 = 0, a2 = 0;
   int64_t b = b0 + lane;
   for (; b + G < b1; b += 2 * G) {
-    const int64_t ja = col[b], jb = col[b + G];
+    const int64_t ja = cb + (int64_t)col[b], jb = cb + (int64_t)col[b + G];
     const MT *va = vals + 9 * b;
     const MT *vb = vals + 9 * (b + G);
     MT xa0 = (MT)x[3 * ja], xa1 = (MT)x[3 * ja + 1], xa2 = (MT)x[3 * ja + 2];
@@ -50,7 +55,7 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int64_t n, const int64_t *__re
     a2 += vb[6] * xb0 + vb[7] * xb1 + vb[8] * xb2;
   }
   if (b < b1) {
-    const int64_t ja = col[b];
+    const int64_t ja = cb + (int64_t)col[b];
     const MT *va = vals + 9 * b;
     MT xa0 = (MT)x[3 * ja], xa1 = (MT)x[3 * ja + 1], xa2 = (MT)x[3 * ja + 2];
     a0 += va[0] * xa0 + va[1] * xa1 + va[2] * xa2;
@@ -504,15 +509,36 @@ int grid_for(int64_t len, int bs, int cap) {
   return (int)g;
 }
 
-template <typename MT, typename VT>
-inch_status launch_spmv(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals, const void *x,
-                        void *y, double s1, double c0, const void *z, double c2, void *acc, double mu,
+// rowbase[i] = first (smallest) column of row i, col16[b] = col[b] - rowbase[i]; flags a
+// row whose span does not fit 16 bits (columns are ascending within a row).
+__global__ void k_col16(int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                        int32_t *__restrict__ rowbase, uint16_t *__restrict__ col16, int32_t *__restrict__ bad) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t b0 = rowptr[i], b1 = rowptr[i + 1];
+  const int32_t base = b1 > b0 ? col[b0] : 0;
+  rowbase[i] = base;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t d = (int64_t)col[b] - base;
+    if (d < 0 || d > 65535) {
+      *bad = 1;
+      col16[b] = 0;
+    } else {
+      col16[b] = (uint16_t)d;
+    }
+  }
+}
+
+template <typename MT, typename VT, typename CT>
+inch_status launch_spmv(int64_t n, const int64_t *rowptr, const CT *col, const int32_t *rowbase, const void *vals,
+                        const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc, double mu,
                         const double *Q, const double *qcoef, int mq, int G, hipStream_t st) {
   const int bs = 256;
   const int64_t grid = (n * G + bs - 1) / bs;
 #define INCH_SPMV_CASE(GG)                                                                                          \
   case GG:                                                                                                          \
-    k_bsr_spmv<MT, VT, GG><<<grid, bs, 0, st>>>(n, rowptr, col, (const MT *)vals, (const VT *)x, (VT *)y, (MT)s1,   \
+    k_bsr_spmv<MT, VT, GG, CT><<<grid, bs, 0, st>>>(n, rowptr, col, rowbase, (const MT *)vals, (const VT *)x,      \
+                                                    (VT *)y, (MT)s1,                                                \
                                                 (MT)c0, (const VT *)z, (MT)c2, (VT *)acc, (MT)mu, Q, qcoef, mq); \
     break;
   switch (G) {
@@ -585,13 +611,55 @@ inch_status inch_bsr_spmv(int64_t n, const int64_t *rowptr, const int32_t *col, 
   INCH_REQUIRE(x != y, "inch_bsr_spmv: x and y must not alias");
   hipStream_t st = (hipStream_t)stream;
   const int G = lanes_per_row > 0 ? lanes_per_row : 16;
+  const int32_t *nb = nullptr;
   if (mdt == INCH_F32 && vdt == INCH_F32)
-    return launch_spmv<float, float>(n, rowptr, col, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
+    return launch_spmv<float, float>(n, rowptr, col, nb, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
   if (mdt == INCH_F32 && vdt == INCH_F64)
-    return launch_spmv<float, double>(n, rowptr, col, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
+    return launch_spmv<float, double>(n, rowptr, col, nb, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
   if (mdt == INCH_F64 && vdt == INCH_F64)
-    return launch_spmv<double, double>(n, rowptr, col, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
+    return launch_spmv<double, double>(n, rowptr, col, nb, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, G, st);
   INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmv: unsupported dtype pair (%d,%d)", mdt, vdt);
+}
+
+inch_status inch_bsr_spmv16(int64_t n, const int64_t *rowptr, const int32_t *rowbase, const uint16_t *col16,
+                            const void *vals, int32_t mdt, int32_t vdt, const void *x, void *y, double s1, double c0,
+                            const void *z, double c2, void *acc, double mu, const double *Q, const double *qcoef,
+                            int32_t m_q, int32_t lanes_per_row, void *stream) {
+  INCH_REQUIRE(n > 0 && rowptr && rowbase && col16 && vals && x && y, "inch_bsr_spmv16: bad args");
+  INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_bsr_spmv16: qcoef needs Q and m_q > 0");
+  INCH_REQUIRE(x != y, "inch_bsr_spmv16: x and y must not alias");
+  hipStream_t st = (hipStream_t)stream;
+  const int G = lanes_per_row > 0 ? lanes_per_row : 16;
+  if (mdt == INCH_F32 && vdt == INCH_F32)
+    return launch_spmv<float, float>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q,
+                                     G, st);
+  if (mdt == INCH_F32 && vdt == INCH_F64)
+    return launch_spmv<float, double>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q,
+                                      G, st);
+  if (mdt == INCH_F64 && vdt == INCH_F64)
+    return launch_spmv<double, double>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef,
+                                       m_q, G, st);
+  INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmv16: unsupported dtype pair (%d,%d)", mdt, vdt);
+}
+
+inch_status inch_bsr_col16(int64_t n, const int64_t *rowptr, const int32_t *col, int32_t *rowbase,
+                           uint16_t *col16, void *stream) {
+  INCH_REQUIRE(n > 0 && rowptr && col && rowbase && col16, "inch_bsr_col16: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  int32_t *flag = nullptr;
+  HIP_TRY(hipMalloc(&flag, sizeof(int32_t)));
+  hipError_t e = hipMemsetAsync(flag, 0, sizeof(int32_t), st);
+  if (e == hipSuccess) {
+    k_col16<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, rowptr, col, rowbase, col16, flag);
+    e = hipGetLastError();
+  }
+  int32_t bad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, flag, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(flag);
+  HIP_TRY(e);
+  if (bad) INCH_FAIL(INCH_ERR_CAPACITY, "inch_bsr_col16: a row spans 65,536 columns or more");
+  return INCH_OK;
 }
 
 inch_status inch_bsr_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, const void *vals, int32_t mdt,

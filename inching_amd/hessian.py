@@ -68,6 +68,16 @@ class BlockHessian:
             raise ValueError("this matrix holds no %s values" % precision)
         return v
 
+    def col16(self):
+        """(rowbase, col16) 16-bit column offsets of the pattern (built once), or None when a
+        row spans too many columns for 16 bits."""
+        if "_col16" not in self.__dict__:
+            try:
+                self._col16 = ops.bsr_col16(self.n_atoms, self.rowptr, self.col)
+            except ops.InchingError:
+                self._col16 = None
+        return self._col16
+
     def ensure(self, precision):
         """Make sure the values of ``precision`` exist (assembled on demand)."""
         if (precision == "f64" and self.vals64 is None) or (precision == "f32" and self.vals32 is None):

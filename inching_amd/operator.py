@@ -42,6 +42,10 @@ class Operator:
             self.lanes = best_config(A, precision)
         else:
             self.lanes = 161 if precision == "mf64" else A.lanes
+        self.blanes, self.c16 = self.lanes, None
+        if precision != "mf64":
+            from .autotune import bsr_launch
+            self.blanes, self.c16 = bsr_launch(A, self.lanes)
         self.n_spmv = 0
         self.n_apply = 0
         dev = A.device
@@ -69,7 +73,8 @@ class Operator:
         if self.vals is None:
             self.A.matfree_spmv(x, y, config=self.lanes, **epi)
         else:
-            ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.lanes, **epi)
+            ops.bsr_spmv(self.A.n_atoms, self.A.rowptr, self.A.col, self.vals, x, y, lanes=self.blanes,
+                         col16=self.c16, **epi)
         if self.timer is not None:
             e1.record()
             epi = "cheb" if (epi.get("acc") is not None or epi.get("z") is not None) else "plain"

@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import F32, F64, call
+from ._lib import F32, F64, InchingError, call  # noqa: F401  (InchingError re-exported)
 
 _DT = {torch.float32: F32, torch.float64: F64}
 
@@ -204,8 +204,19 @@ def hessian_assemble(xyz, rc, plus_i, rowptr, col, want64=True, want32=True, str
     return v64, v32, diag
 
 
+def bsr_col16(n_atoms, rowptr, col):
+    """(rowbase int32 [n], col16 uint16 [nnzb]): 16-bit column offsets from each row's first
+    column (raises InchingError when a row spans 65,536 columns or more)."""
+    rowbase = torch.empty(n_atoms, dtype=torch.int32, device=col.device)
+    col16 = torch.empty(max(1, col.numel()), dtype=torch.uint16, device=col.device)
+    call("inch_bsr_col16", n_atoms, P(rowptr), P(col), P(rowbase), P(col16), stream_ptr(None))
+    return rowbase, col16[:col.numel()]
+
+
 def bsr_spmv(n_atoms, rowptr, col, vals, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None,
-             qcoef=None, lanes=16, stream=None):
+             qcoef=None, lanes=16, stream=None, col16=None):
+    """y = s1 (A x - c0 x) - c2 z (+ sum_m qcoef[m] Q[m]); acc += mu y (all terms optional).
+    ``col16`` = (rowbase, col16) selects the 16-bit column-offset kernel (col is then unused)."""
     nd = 3 * n_atoms
     for t in (x, y, z, acc):
         if t is not None and t.numel() != nd:
@@ -223,6 +234,11 @@ def bsr_spmv(n_atoms, rowptr, col, vals, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, a
         mq = Q.shape[0]
         if qcoef.numel() < mq:
             raise ValueError("qcoef too short")
+    if col16 is not None:
+        call("inch_bsr_spmv16", n_atoms, P(rowptr), P(col16[0]), P(col16[1]), P(vals), dt(vals), vd, P(x), P(y),
+             float(s1), float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(lanes),
+             stream_ptr(stream))
+        return
     call("inch_bsr_spmv", n_atoms, P(rowptr), P(col), P(vals), dt(vals), vd, P(x), P(y), float(s1), float(c0),
          P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(lanes), stream_ptr(stream))
 

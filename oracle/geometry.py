@@ -5,7 +5,7 @@ Reference: the k-d tree ball queries of ``InchingLiteInteger/Fuel/Coordinate/T1.
 (:291-299): atom j is a neighbour of atom i iff |x_j - x_i|^2 <= rc^2. Self is
 included (degree counts it, ``Fuel/Coordinate/T1.py:236-245``).
 
-Pinned rule used by this build (the reference leaves the floating-point evaluation
+Pinned rule used here (the reference leaves the floating-point evaluation
 to scipy/CuPy): d^2 = (dx*dx + dy*dy) + dz*dz in float64 with dx = x_j - x_i, no
 fused multiply-add, compared with rc*rc (rc*rc evaluated in float64).
 """

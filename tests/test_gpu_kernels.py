@@ -323,6 +323,37 @@ def test_cgs2_reorth_fused(cuda, J, n):
     assert np.max(np.abs(V @ Vd[J].cpu().numpy())) < 1e-13
 
 
+def test_spmv_col16_bitwise_equal(mat):
+    """16-bit column offsets (row base + offset) give bitwise the int32-column result for every
+    lane count and dtype pair; a row spanning more than 16 bits is rejected."""
+    from inching_amd import ops
+    from inching_amd._lib import InchingError
+    A, Bs = mat
+    _print("case: bsr col16")
+    rb, c16 = ops.bsr_col16(A.n_atoms, A.rowptr, A.col)
+    rp = A.rowptr.cpu().numpy()
+    col = A.col.cpu().numpy().astype(np.int64)
+    rows = np.repeat(np.arange(A.n_atoms), np.diff(rp))
+    assert np.array_equal(rb.cpu().numpy()[rows] + c16.cpu().numpy().astype(np.int64), col)
+    rng = np.random.default_rng(4)
+    x, z, acc0 = (rng.standard_normal(A.shape[0]) for _ in range(3))
+    for vals, prec in ((A.vals32, torch.float32), (A.vals32, torch.float64), (A.vals64, torch.float64)):
+        for lanes in (4, 8, 16, 32, 64):
+            outs = []
+            for c in (None, (rb, c16)):
+                y = torch.empty(A.shape[0], dtype=prec, device="cuda")
+                acc = _dev(acc0, prec)
+                ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, _dev(x, prec), y, s1=0.3, c0=2.0, z=_dev(z, prec),
+                             c2=1.0, acc=acc, mu=0.5, lanes=lanes, col16=c)
+                outs.append((y.cpu().numpy(), acc.cpu().numpy()))
+            assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1]), (lanes, prec)
+    # a row whose columns span more than 65,535
+    ip = torch.tensor([0, 2, 3], dtype=torch.int64, device="cuda")
+    ix = torch.tensor([0, 70000, 1], dtype=torch.int32, device="cuda")
+    with pytest.raises(InchingError):
+        ops.bsr_col16(2, ip, ix)
+
+
 def test_matfree_launch_configs_agree(mat):
     from inching_amd import ops
     from inching_amd.autotune import MATFREE_CONFIGS
