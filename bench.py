@@ -273,7 +273,8 @@ def kernel_key(kind, A):
     from inching_amd.autotune import best_config
     cfg = best_config(A, kind)
     if kind == "mf64":
-        return "k_matfree_epi<double,%d,%d>" % (cfg // 10, cfg % 10)
+        c = cfg % 1000
+        return "k_matfree_epi<double,%d,%d,%s>" % (c // 10, c % 10, "unsigned short" if cfg >= 1000 else "int")
     t = "float" if kind == "f32" else "double"
     if cfg >= 200:
         return "k_bsr_spmv<%s,%s,%d,unsigned short>" % (t, t, cfg - 200)

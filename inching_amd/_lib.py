@@ -38,6 +38,8 @@ SIGNATURES = {
     "inch_hessian_assemble_pbc": (_STATUS, [_i64, _vp, _dbl, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp]),
     "inch_anm_matfree_pbc": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _i32, _vp, _vp, _i32, _dbl, _dbl,
                                        _vp, _dbl, _vp, _dbl, _vp, _vp, _i32, _vp]),
+    "inch_anm_matfree_spmv16": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp,
+                                          _dbl, _vp, _vp, _i32, _i32, _vp]),
     "inch_bsr_spmv16": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp, _dbl,
                                   _vp, _vp, _i32, _i32, _vp]),
     "inch_bsr_col16": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp]),

@@ -83,7 +83,9 @@ def best_config(A, precision, reps=5):
     acc = torch.zeros_like(x)
     times = {}
     if kind == "matfree":
-        for cfg in MATFREE_CONFIGS:
+        # int32 columns: lanes*10+unroll; 16-bit column offsets: 1000 + lanes*10+unroll
+        codes = MATFREE_CONFIGS + (tuple(1000 + c for c in MATFREE_CONFIGS) if A.col16() is not None else ())
+        for cfg in codes:
             times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
                                reps)
     else:

@@ -204,10 +204,11 @@ __global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__rest
 //   out = s1 * (A x - c0 * x) - c2 * z (+ Q term); y = out; acc += mu * out.
 // G lanes per row; each lane handles U neighbours per iteration with independent loads
 // (memory-level parallelism for the dependent index -> coordinate/vector gathers).
-template <typename VT, int G, int U>
+template <typename VT, int G, int U, typename CT>
 __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__restrict__ xyz,
                                                       const int64_t *__restrict__ rowptr,
-                                                      const int32_t *__restrict__ col, double plus_i,
+                                                      const CT *__restrict__ col,
+                                                      const int32_t *__restrict__ rowbase, double plus_i,
                                                       const VT *__restrict__ x, VT *__restrict__ y, double s1,
                                                       double c0, const VT *__restrict__ z, double c2,
                                                       VT *__restrict__ acc, double mu, const double *__restrict__ Q,
@@ -219,10 +220,11 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
   const double vi0 = (double)x[3 * row], vi1 = (double)x[3 * row + 1], vi2 = (double)x[3 * row + 2];
   double a0 = 0, a1 = 0, a2 = 0;
   const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
+  const int64_t cb = rowbase ? (int64_t)rowbase[row] : 0;   // 16-bit column offsets: row base
   for (int64_t b = b0 + lane; b < b1; b += G * U) {
     int64_t jj[U];
 #pragma unroll
-    for (int q = 0; q < U; ++q) jj[q] = (b + q * G < b1) ? (int64_t)col[b + q * G] : row;
+    for (int q = 0; q < U; ++q) jj[q] = (b + q * G < b1) ? cb + (int64_t)col[b + q * G] : row;
 #pragma unroll
     for (int q = 0; q < U; ++q) {
       const int64_t j = jj[q];
@@ -565,23 +567,25 @@ inch_status launch_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, co
   return INCH_OK;
 }
 
-template <typename VT, int G, int U>
-void launch_matfree(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col, double plus_i,
-                           const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc,
-                           double mu, const double *Q, const double *qcoef, int m_q, hipStream_t st) {
+template <typename VT, int G, int U, typename CT>
+void launch_matfree(int64_t n, const double *xyz, const int64_t *rowptr, const CT *col, const int32_t *rowbase,
+                    double plus_i, const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc,
+                    double mu, const double *Q, const double *qcoef, int m_q, hipStream_t st) {
   const int bs = 256;
   const int64_t grid = (n * G + bs - 1) / bs;
-  k_matfree_epi<VT, G, U><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, plus_i, (const VT *)x, (VT *)y, s1, c0,
-                                               (const VT *)z, c2, (VT *)acc, mu, Q, qcoef, m_q);
+  k_matfree_epi<VT, G, U, CT><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, rowbase, plus_i, (const VT *)x, (VT *)y, s1,
+                                                   c0, (const VT *)z, c2, (VT *)acc, mu, Q, qcoef, m_q);
 }
 
-template <typename VT>
-bool dispatch_matfree(int code, int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
-                             double plus_i, const void *x, void *y, double s1, double c0, const void *z, double c2,
-                             void *acc, double mu, const double *Q, const double *qcoef, int m_q, hipStream_t st) {
-#define INCH_MF(GG, UU)                                                                                           \
-  case GG * 10 + UU:                                                                                              \
-    launch_matfree<VT, GG, UU>(n, xyz, rowptr, col, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, st); \
+template <typename VT, typename CT>
+bool dispatch_matfree(int code, int64_t n, const double *xyz, const int64_t *rowptr, const CT *col,
+                      const int32_t *rowbase, double plus_i, const void *x, void *y, double s1, double c0,
+                      const void *z, double c2, void *acc, double mu, const double *Q, const double *qcoef, int m_q,
+                      hipStream_t st) {
+#define INCH_MF(GG, UU)                                                                                        \
+  case GG * 10 + UU:                                                                                           \
+    launch_matfree<VT, GG, UU, CT>(n, xyz, rowptr, col, rowbase, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, \
+                                   m_q, st);                                                                   \
     return true;
   switch (code) {
     INCH_MF(4, 1)
@@ -693,11 +697,14 @@ inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *r
   INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_spmv: qcoef needs Q");
   hipStream_t st = (hipStream_t)stream;
   const int code = config > 0 ? config : 161;
+  const int32_t *nb = nullptr;
   bool ok;
   if (vdt == INCH_F64)
-    ok = dispatch_matfree<double>(code, n, xyz, rowptr, col, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, st);
+    ok = dispatch_matfree<double>(code, n, xyz, rowptr, col, nb, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q,
+                                  st);
   else if (vdt == INCH_F32)
-    ok = dispatch_matfree<float>(code, n, xyz, rowptr, col, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q, st);
+    ok = dispatch_matfree<float>(code, n, xyz, rowptr, col, nb, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q,
+                                 st);
   else
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: bad dtype");
   if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: config %d is not lanes*10+unroll with lanes in "
@@ -729,6 +736,29 @@ inch_status inch_anm_matfree_pbc(int64_t n, const double *xyz, const int64_t *ro
                                               qcoef, m_q);
   else
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_pbc: bad dtype");
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_anm_matfree_spmv16(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *rowbase,
+                                    const uint16_t *col16, double plus_i, int32_t vdt, const void *x, void *y,
+                                    double s1, double c0, const void *z, double c2, void *acc, double mu,
+                                    const double *Q, const double *qcoef, int32_t m_q, int32_t config, void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && rowptr && rowbase && col16 && x && y && x != y, "inch_anm_matfree_spmv16: bad args");
+  INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_spmv16: qcoef needs Q");
+  hipStream_t st = (hipStream_t)stream;
+  const int code = config > 0 ? config : 161;
+  bool ok;
+  if (vdt == INCH_F64)
+    ok = dispatch_matfree<double>(code, n, xyz, rowptr, col16, rowbase, plus_i, x, y, s1, c0, z, c2, acc, mu, Q,
+                                  qcoef, m_q, st);
+  else if (vdt == INCH_F32)
+    ok = dispatch_matfree<float>(code, n, xyz, rowptr, col16, rowbase, plus_i, x, y, s1, c0, z, c2, acc, mu, Q,
+                                 qcoef, m_q, st);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv16: bad dtype");
+  if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv16: config %d is not lanes*10+unroll with lanes in "
+                                       "{4,8,16,32} and unroll in {1,2}", code);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

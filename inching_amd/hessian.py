@@ -112,11 +112,15 @@ class BlockHessian:
 
     def matfree_spmv(self, x, y, config=0, **epi):
         """Matrix-free float64 product with the fused epilogue (``config``: autotuner code
-        lanes*10+unroll, 0 = default; periodic matrices use their own kernel)."""
+        lanes*10+unroll, plus 1000 for 16-bit column offsets, 0 = default; periodic matrices
+        use their own kernel)."""
         if self.images is not None:
             ops.matfree_pbc(self.xyz, self.rowptr, self.col, self.images, self.box, self.plus_i, x, y, **epi)
+        elif config >= 1000 and self.col16() is not None:
+            ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config - 1000,
+                             col16=self.col16(), **epi)
         else:
-            ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config, **epi)
+            ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config % 1000, **epi)
 
     def matfree(self, X):
         """FP64 (H + PlusI I) X recomputed from coordinates (X [3n] or [3n, r])."""

@@ -272,8 +272,9 @@ def matfree_apply(xyz, rowptr, col, plus_i, X, Y, stream=None):
 
 
 def matfree_spmv(xyz, rowptr, col, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None,
-                 qcoef=None, config=0, stream=None):
-    """Matrix-free ANM product with the fused epilogue (see bsr_spmv)."""
+                 qcoef=None, config=0, stream=None, col16=None):
+    """Matrix-free ANM product with the fused epilogue (see bsr_spmv); ``col16`` = (rowbase,
+    col16) selects 16-bit column offsets."""
     n = xyz.shape[0]
     nd = 3 * n
     for t in (x, y, z, acc):
@@ -288,6 +289,11 @@ def matfree_spmv(xyz, rowptr, col, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0,
         if Q is None or Q.dtype != torch.float64 or Q.shape[1] != nd:
             raise ValueError("Q must be float64 [m, 3n]")
         mq = Q.shape[0]
+    if col16 is not None:
+        call("inch_anm_matfree_spmv16", n, P(xyz), P(rowptr), P(col16[0]), P(col16[1]), float(plus_i), vd, P(x), P(y),
+             float(s1), float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(config),
+             stream_ptr(stream))
+        return
     call("inch_anm_matfree_spmv", n, P(xyz), P(rowptr), P(col), float(plus_i), vd, P(x), P(y), float(s1),
          float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(config), stream_ptr(stream))
 

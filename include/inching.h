@@ -179,6 +179,12 @@ inch_status inch_anm_matfree_pbc(int64_t n, const double *xyz, const int64_t *ro
                                  double c2, void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
                                  void *stream);
 
+/* inch_anm_matfree_spmv with 16-bit column offsets (rowbase / col16 from inch_bsr_col16). */
+inch_status inch_anm_matfree_spmv16(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *rowbase,
+                                    const uint16_t *col16, double plus_i, int32_t vdt, const void *x, void *y,
+                                    double s1, double c0, const void *z, double c2, void *acc, double mu,
+                                    const double *Q, const double *qcoef, int32_t m_q, int32_t config, void *stream);
+
 /* inch_bsr_spmv with 16-bit column offsets (same reference interfaces): column j of block b
  * in row i is rowbase[i] + col16[b]; inch_bsr_col16 builds both from the int32 columns and
  * fails with INCH_ERR_CAPACITY when a row spans 65,536 columns or more (syncs). */

@@ -369,5 +369,10 @@ def test_matfree_launch_configs_agree(mat):
         out = 0.3 * (Bs @ x - 2.0 * x) - z
         assert np.allclose(y.cpu().numpy(), out, rtol=1e-12, atol=1e-12), cfg
         assert np.allclose(acc.cpu().numpy(), acc0 + 0.5 * out, rtol=1e-12, atol=1e-12)
+        # 16-bit column offsets (config + 1000): bitwise the same result
+        y16 = torch.empty_like(y)
+        acc16 = _dev(acc0)
+        A.matfree_spmv(_dev(x), y16, config=1000 + cfg, s1=0.3, c0=2.0, z=_dev(z), c2=1.0, acc=acc16, mu=0.5)
+        assert torch.equal(y16, y) and torch.equal(acc16, acc), cfg
         if ref is None:
             ref = y.cpu().numpy()
