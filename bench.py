@@ -276,9 +276,11 @@ def kernel_key(kind, A):
         c = cfg % 1000
         return "k_matfree_epi<double,%d,%d,%s>" % (c // 10, c % 10, "unsigned short" if cfg >= 1000 else "int")
     t = "float" if kind == "f32" else "double"
+    if cfg >= 1200:
+        return "k_bsr_spmv<%s,%s,%d,unsigned short,true>" % (t, t, cfg - 1200)
     if cfg >= 200:
-        return "k_bsr_spmv<%s,%s,%d,unsigned short>" % (t, t, cfg - 200)
-    return "k_bsr_spmv<%s,%s,%d,int>" % (t, t, cfg)
+        return "k_bsr_spmv<%s,%s,%d,unsigned short,false>" % (t, t, cfg - 200)
+    return "k_bsr_spmv<%s,%s,%d,int,false>" % (t, t, cfg)
 
 
 def measured_traffic(key):
@@ -288,9 +290,10 @@ def measured_traffic(key):
     try:
         with open(path) as f:
             for line in f:
-                p = line.split()
-                if len(p) == 3 and p[0] == "traffic" and p[1] == key:
-                    return int(p[2])
+                if line.startswith("traffic "):
+                    name, value = line[len("traffic "):].strip().rsplit(None, 1)
+                    if name == key:
+                        return int(value)
     except OSError:
         pass
     return None

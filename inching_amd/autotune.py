@@ -97,9 +97,10 @@ def best_config(A, precision, reps=5):
         c16 = A.col16()
         if c16 is not None:
             for lanes in BSR_LANES:
-                times[200 + lanes] = _time(lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, s1=0.1,
-                                                                c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, lanes=lanes,
-                                                                col16=c16), reps)
+                for base, arg in ((200, lanes), (1200, lanes + 1000)):       # 1200+: XCD-contiguous rows
+                    times[base + lanes] = _time(lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, s1=0.1,
+                                                                     c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, lanes=arg,
+                                                                     col16=c16), reps)
     best = min(times, key=times.get)
     _cache[k] = {"best": int(best), "ms": {str(c): round(t, 5) for c, t in times.items()}}
     _save()
@@ -107,8 +108,11 @@ def best_config(A, precision, reps=5):
 
 
 def bsr_launch(A, code):
-    """(lanes, col16 or None) of a block-SpMV config code."""
+    """(lanes argument, col16 or None) of a block-SpMV config code: lanes (int32 columns),
+    200 + lanes (16-bit columns), 1200 + lanes (16-bit columns, XCD-contiguous rows)."""
     code = int(code)
+    if code >= 1200:
+        return code - 1200 + 1000, A.col16()
     if code >= 200:
         return code - 200, A.col16()
     return code, None

@@ -26,7 +26,17 @@ constexpr int kRedBlocks = 512;  // fixed grid of the two-phase reductions (dete
 // Column indices: absolute int32 (CT = int32_t, rowbase = nullptr), or 16-bit offsets from
 // the row's first column (CT = uint16_t, rowbase[row] = that column), 2 B per block instead
 // of 4 B when every row of the RCM band spans fewer than 65,536 columns.
-template <typename MT, typename VT, int G, typename CT>
+// XCD-aware row mapping: workgroups are dealt round-robin to the 8 XCDs, so workgroup w runs
+// on XCD w % 8; remapping w to a contiguous range per XCD keeps each XCD's rows (and the x
+// entries they gather) in one band, which its own L2 then holds.
+__device__ __forceinline__ int64_t xcd_block(bool remap) {
+  const int64_t bid = blockIdx.x;
+  if (!remap) return bid;
+  const int64_t nb = gridDim.x, q = nb >> 3, r = nb & 7, xcd = bid & 7, idx = bid >> 3;
+  return xcd * q + (xcd < r ? xcd : r) + idx;   // a bijection of [0, nb)
+}
+
+template <typename MT, typename VT, int G, typename CT, bool XCD = false>
 __global__ void __launch_bounds__(256) k_bsr_spmv(int64_t n, const int64_t *__restrict__ rowptr,
                                                    const CT *__restrict__ col, const int32_t *__restrict__ rowbase,
                                                    const MT *__restrict__ vals,
@@ -35,7 +45,7 @@ __global__ void __launch_bounds__(256) k_bsr_spmv(int64_t n, const int64_t *__re
                                                    const double *__restrict__ Q, const double *__restrict__ qcoef,
                                                    int mq) {
   const int lane = threadIdx.x & (G - 1);
-  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int64_t row = (xcd_block(XCD) * (int64_t)blockDim.x + threadIdx.x) / G;
   if (row >= n) return;  // whole groups leave together (a group never spans two rows)
   const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
   const int64_t cb = rowbase ? (int64_t)rowbase[row] : 0;
@@ -531,7 +541,7 @@ __global__ void k_col16(int64_t n, const int64_t *__restrict__ rowptr, const int
   }
 }
 
-template <typename MT, typename VT, typename CT>
+template <typename MT, typename VT, typename CT, bool XCD = false>
 inch_status launch_spmv(int64_t n, const int64_t *rowptr, const CT *col, const int32_t *rowbase, const void *vals,
                         const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc, double mu,
                         const double *Q, const double *qcoef, int mq, int G, hipStream_t st) {
@@ -539,7 +549,7 @@ inch_status launch_spmv(int64_t n, const int64_t *rowptr, const CT *col, const i
   const int64_t grid = (n * G + bs - 1) / bs;
 #define INCH_SPMV_CASE(GG)                                                                                          \
   case GG:                                                                                                          \
-    k_bsr_spmv<MT, VT, GG, CT><<<grid, bs, 0, st>>>(n, rowptr, col, rowbase, (const MT *)vals, (const VT *)x,      \
+    k_bsr_spmv<MT, VT, GG, CT, XCD><<<grid, bs, 0, st>>>(n, rowptr, col, rowbase, (const MT *)vals, (const VT *)x,      \
                                                     (VT *)y, (MT)s1,                                                \
                                                 (MT)c0, (const VT *)z, (MT)c2, (VT *)acc, (MT)mu, Q, qcoef, mq); \
     break;
@@ -633,16 +643,24 @@ inch_status inch_bsr_spmv16(int64_t n, const int64_t *rowptr, const int32_t *row
   INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_bsr_spmv16: qcoef needs Q and m_q > 0");
   INCH_REQUIRE(x != y, "inch_bsr_spmv16: x and y must not alias");
   hipStream_t st = (hipStream_t)stream;
-  const int G = lanes_per_row > 0 ? lanes_per_row : 16;
+  // lanes_per_row + 1000: XCD-contiguous row mapping
+  const bool xcd = lanes_per_row >= 1000;
+  const int G = lanes_per_row % 1000 > 0 ? lanes_per_row % 1000 : 16;
   if (mdt == INCH_F32 && vdt == INCH_F32)
-    return launch_spmv<float, float>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q,
-                                     G, st);
+    return xcd ? launch_spmv<float, float, uint16_t, true>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc,
+                                                            mu, Q, qcoef, m_q, G, st)
+               : launch_spmv<float, float>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef,
+                                           m_q, G, st);
   if (mdt == INCH_F32 && vdt == INCH_F64)
-    return launch_spmv<float, double>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, m_q,
-                                      G, st);
+    return xcd ? launch_spmv<float, double, uint16_t, true>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2,
+                                                             acc, mu, Q, qcoef, m_q, G, st)
+               : launch_spmv<float, double>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef,
+                                            m_q, G, st);
   if (mdt == INCH_F64 && vdt == INCH_F64)
-    return launch_spmv<double, double>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef,
-                                       m_q, G, st);
+    return xcd ? launch_spmv<double, double, uint16_t, true>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2,
+                                                              acc, mu, Q, qcoef, m_q, G, st)
+               : launch_spmv<double, double>(n, rowptr, col16, rowbase, vals, x, y, s1, c0, z, c2, acc, mu, Q, qcoef,
+                                             m_q, G, st);
   INCH_FAIL(INCH_ERR_INVALID, "inch_bsr_spmv16: unsupported dtype pair (%d,%d)", mdt, vdt);
 }
 

@@ -187,7 +187,8 @@ inch_status inch_anm_matfree_spmv16(int64_t n, const double *xyz, const int64_t 
 
 /* inch_bsr_spmv with 16-bit column offsets (same reference interfaces): column j of block b
  * in row i is rowbase[i] + col16[b]; inch_bsr_col16 builds both from the int32 columns and
- * fails with INCH_ERR_CAPACITY when a row spans 65,536 columns or more (syncs). */
+ * fails with INCH_ERR_CAPACITY when a row spans 65,536 columns or more (syncs).
+ * lanes_per_row + 1000 maps each XCD's workgroups to one contiguous range of rows. */
 inch_status inch_bsr_spmv16(int64_t n, const int64_t *rowptr, const int32_t *rowbase, const uint16_t *col16,
                             const void *vals, int32_t mdt, int32_t vdt, const void *x, void *y, double s1, double c0,
                             const void *z, double c2, void *acc, double mu, const double *Q, const double *qcoef,

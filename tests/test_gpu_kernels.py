@@ -340,13 +340,14 @@ def test_spmv_col16_bitwise_equal(mat):
     for vals, prec in ((A.vals32, torch.float32), (A.vals32, torch.float64), (A.vals64, torch.float64)):
         for lanes in (4, 8, 16, 32, 64):
             outs = []
-            for c in (None, (rb, c16)):
+            for c, la in ((None, lanes), ((rb, c16), lanes), ((rb, c16), lanes + 1000)):   # +1000: XCD rows
                 y = torch.empty(A.shape[0], dtype=prec, device="cuda")
                 acc = _dev(acc0, prec)
                 ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, _dev(x, prec), y, s1=0.3, c0=2.0, z=_dev(z, prec),
-                             c2=1.0, acc=acc, mu=0.5, lanes=lanes, col16=c)
+                             c2=1.0, acc=acc, mu=0.5, lanes=la, col16=c)
                 outs.append((y.cpu().numpy(), acc.cpu().numpy()))
-            assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1]), (lanes, prec)
+            for o in outs[1:]:
+                assert np.array_equal(outs[0][0], o[0]) and np.array_equal(outs[0][1], o[1]), (lanes, prec)
     # a row whose columns span more than 65,535
     ip = torch.tensor([0, 2, 3], dtype=torch.int64, device="cuda")
     ix = torch.tensor([0, 70000, 1], dtype=torch.int32, device="cuda")
