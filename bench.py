@@ -251,16 +251,28 @@ def run_warm(args):
     print(json.dumps(out))
 
 
-def kernel_bytes(kind, nnzb, n_atoms, epi):
-    """Algorithmic bytes of one operator launch (values + vectors; no index bytes)."""
+def kernel_bytes(kind, A, epi):
+    """Algorithmic bytes of one operator launch of the tuned kernel: stored values + vectors
+    for the block SpMV (no index bytes); for the matrix-free operator, whose neighbour indices
+    and coordinates are the data it streams, the index layout the tuned kernel reads + the
+    coordinates + the vectors."""
+    from inching_amd.autotune import best_config
+    nnzb, n_atoms = A.nnzb, A.n_atoms
     nd = 3 * n_atoms
     if kind == "f32":
         s, sv = 4, 4 if epi == "cheb" else 8
         return 9 * s * nnzb + (5 if epi == "cheb" else 2) * nd * sv
     if kind == "f64":
         return 9 * 8 * nnzb + (5 if epi == "cheb" else 2) * nd * 8
-    # matrix-free: neighbour indices + coordinates + vectors
-    return 4 * nnzb + 8 * (n_atoms + 1) + 24 * n_atoms + (5 if epi == "cheb" else 2) * nd * 8
+    vec = (5 if epi == "cheb" else 2) * nd * 8
+    flags = best_config(A, kind) // 1000
+    if flags & 4:       # cluster layout: entries + cl_ptr + base + cluster_atoms
+        cl = A.clusters()
+        ncl = cl[2].numel()
+        return 4 * cl[3].numel() + 8 * (ncl + 1) + 4 * ncl + 32 * ncl + 24 * n_atoms + vec
+    if flags & 1:       # 16-bit column offsets + row bases
+        return 2 * nnzb + 8 * (n_atoms + 1) + 4 * n_atoms + 24 * n_atoms + vec
+    return 4 * nnzb + 8 * (n_atoms + 1) + 24 * n_atoms + vec
 
 
 KERNEL_DESC = {"f32": "block SpMV, fp32 values, Chebyshev epilogue",
@@ -273,8 +285,11 @@ def kernel_key(kind, A):
     from inching_amd.autotune import best_config
     cfg = best_config(A, kind)
     if kind == "mf64":
-        c = cfg % 1000
-        return "k_matfree_epi<double,%d,%d,%s>" % (c // 10, c % 10, "unsigned short" if cfg >= 1000 else "int")
+        c, f = cfg % 1000, cfg // 1000
+        if f & 4:
+            return "k_matfree_cl<double,%d,%s>" % (c, "true" if f & 2 else "false")
+        return "k_matfree_epi<double,%d,%d,%s,%s>" % (c // 10, c % 10, "unsigned short" if f & 1 else "int",
+                                                       "true" if f & 2 else "false")
     t = "float" if kind == "f32" else "double"
     if cfg >= 1200:
         return "k_bsr_spmv<%s,%s,%d,unsigned short,true>" % (t, t, cfg - 1200)
@@ -363,7 +378,7 @@ def main():
             return None
         (kind, epi), (ms_total, cnt) = max(per.items(), key=lambda kv: kv[1][0])
         avg_ms = ms_total / cnt
-        by = kernel_bytes(kind, nnzb, n_atoms, epi)
+        by = kernel_bytes(kind, A, epi)
         ach = by / (avg_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_key(kind, A)),

@@ -18,6 +18,7 @@ from . import ops
 
 BSR_LANES = (4, 8, 16, 32, 64)
 MATFREE_CONFIGS = (41, 42, 81, 82, 161, 162, 321, 322)
+CLUSTER_CONFIGS = (1, 2, 4)   # list entries per lane iteration
 
 _cache = {}
 _loaded = False
@@ -83,8 +84,12 @@ def best_config(A, precision, reps=5):
     acc = torch.zeros_like(x)
     times = {}
     if kind == "matfree":
-        # int32 columns: lanes*10+unroll; 16-bit column offsets: 1000 + lanes*10+unroll
-        codes = MATFREE_CONFIGS + (tuple(1000 + c for c in MATFREE_CONFIGS) if A.col16() is not None else ())
+        # lanes*10+unroll, + 1000 for 16-bit column offsets, + 2000 for XCD-contiguous rows,
+        # + 4000 for the cluster layout
+        flags = (0, 2000) + ((1000, 3000) if A.col16() is not None else ())
+        codes = tuple(f + c for f in flags for c in MATFREE_CONFIGS)
+        if A.clusters() is not None:
+            codes += tuple(f + c for f in (4000, 6000) for c in CLUSTER_CONFIGS)
         for cfg in codes:
             times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
                                reps)

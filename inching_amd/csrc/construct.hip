@@ -378,6 +378,180 @@ __global__ void __launch_bounds__(256) k_assemble(int64_t n, const double *__res
   }
 }
 
+// ---- cluster layout of the matrix-free operator --------------------------------------
+// Atoms are grouped into clusters of kClusterAtoms that are compact in space: cells of
+// cell_xy x cell_xy x cell_z, numbered column-major (x, y column, then z inside the column),
+// atoms in a cell by index; cluster c holds atoms [4c, 4c + 4) of that order. Each cluster
+// lists the union of its atoms' neighbour rows once, as (j, mask) with mask bit t set when j
+// is a neighbour of the cluster's atom t (j itself excluded), so the product gathers a
+// neighbour's coordinates and vector entries once per cluster instead of once per atom.
+constexpr int kClusterAtoms = 4;
+constexpr int kMaxUnion = 4096;  // row entries one cluster may gather (LDS sort capacity)
+
+struct CGrid {
+  double lo[3];
+  double inv[3];  // 1 / cell size per axis
+  int64_t dim[3];
+};
+
+__global__ void k_cl_cell(int64_t n, const double *__restrict__ xyz, CGrid g, int32_t *__restrict__ cell_of,
+                          int32_t *__restrict__ count) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t c[3];
+  for (int a = 0; a < 3; ++a) c[a] = cell_coord(xyz[3 * i + a], g.lo[a], g.inv[a], g.dim[a]);
+  const int64_t id = (c[0] * g.dim[1] + c[1]) * g.dim[2] + c[2];
+  cell_of[i] = (int32_t)id;
+  atomicAdd(&count[id], 1);
+}
+
+// One thread per cell: insertion sort of the cell's atoms by index (cells hold a few atoms),
+// which makes the order independent of the atomic fill order.
+__global__ void k_cell_isort(int64_t ncell, const int32_t *__restrict__ start, int32_t *__restrict__ atoms) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int32_t b = start[c], e = start[c + 1];
+  for (int32_t p = b + 1; p < e; ++p) {
+    const int32_t v = atoms[p];
+    int32_t q = p - 1;
+    while (q >= b && atoms[q] > v) {
+      atoms[q + 1] = atoms[q];
+      --q;
+    }
+    atoms[q + 1] = v;
+  }
+}
+
+__global__ void k_cl_atoms(int64_t n, int64_t slots, const int32_t *__restrict__ atoms,
+                           int32_t *__restrict__ cluster_atoms) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q < slots) cluster_atoms[q] = q < n ? atoms[q] : -1;
+}
+
+// One block per cluster. The rows of its atoms are gathered as keys (j << 3 | t), sorted
+// (bitonic, LDS), and each run of equal j becomes one entry with the OR of its atom bits.
+// Count pass (FILL = false): counts[c] = entries, base[c] = smallest j; overflow bit 1 =
+// more than kMaxUnion row entries, bit 2 = the union spans 2^24 columns or more.
+// Fill pass: entries[cl_ptr[c] + r] = ((j - base[c]) << 8) | mask, ascending j.
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const int64_t *__restrict__ rowptr,
+                                                   const int32_t *__restrict__ col,
+                                                   const int32_t *__restrict__ cluster_atoms,
+                                                   const int64_t *__restrict__ cl_ptr, int32_t *__restrict__ base,
+                                                   int64_t *__restrict__ counts, uint32_t *__restrict__ entries,
+                                                   int32_t *__restrict__ overflow) {
+  __shared__ uint64_t key[kMaxUnion];
+  __shared__ int32_t val[kMaxUnion];
+  __shared__ int32_t part[256];
+  __shared__ int64_t rb[kClusterAtoms], rl[kClusterAtoms + 1];
+  __shared__ int32_t ca[kClusterAtoms];
+  __shared__ int32_t s_min, s_max;
+  const int64_t c = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    int64_t acc = 0;
+    for (int t = 0; t < kClusterAtoms; ++t) {
+      const int32_t a = cluster_atoms[kClusterAtoms * c + t];
+      ca[t] = a;
+      const int64_t b = a >= 0 ? rowptr[a] : 0, e = a >= 0 ? rowptr[a + 1] : 0;
+      rb[t] = b;
+      rl[t] = acc;
+      acc += e - b;
+    }
+    rl[kClusterAtoms] = acc;
+    s_min = 0;
+    s_max = 0;
+  }
+  __syncthreads();
+  const int64_t total64 = rl[kClusterAtoms];
+  if (total64 > kMaxUnion) {  // uniform over the block
+    if (tid == 0) {
+      atomicOr(overflow, 1);
+      if (!FILL) counts[c] = 0;
+    }
+    return;
+  }
+  const int total = (int)total64;
+  int p2 = 1;
+  while (p2 < total) p2 <<= 1;
+  for (int e = tid; e < p2; e += blockDim.x) {
+    uint64_t k = ~0ull;
+    if (e < total) {
+      int t = 0;
+      while (e >= rl[t + 1]) ++t;
+      const int32_t j = col[rb[t] + (e - rl[t])];
+      k = ((uint64_t)(uint32_t)j << 3) | (uint64_t)t;
+    }
+    key[e] = k;
+  }
+  __syncthreads();
+  for (int k = 2; k <= p2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = tid; t < p2; t += blockDim.x) {
+        const int ixj = t ^ j;
+        if (ixj > t) {
+          const bool up = (t & k) == 0;
+          const uint64_t a = key[t], b = key[ixj];
+          if ((a > b) == up) {
+            key[t] = b;
+            key[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // runs of equal j: the run's first element carries the mask (0 = nothing to emit)
+  const int chunk = (total + 255) / 256;
+  const int e0 = tid * chunk, e1 = min(total, e0 + chunk);
+  int cnt = 0;
+  for (int e = e0; e < e1; ++e) {
+    const uint32_t j = (uint32_t)(key[e] >> 3);
+    int32_t m = 0;
+    if (e == 0 || (uint32_t)(key[e - 1] >> 3) != j) {
+      for (int f = e; f < total && (uint32_t)(key[f] >> 3) == j; ++f) {
+        const int t = (int)(key[f] & 7);
+        if ((uint32_t)ca[t] != j) m |= 1 << t;
+      }
+    }
+    val[e] = m;
+    cnt += m != 0;
+  }
+  part[tid] = cnt;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  const int before = part[tid] - cnt, emitted = part[255];
+  if (!FILL) {
+    if (cnt > 0) {
+      int first = -1, last = -1;
+      for (int e = e0; e < e1; ++e)
+        if (val[e]) {
+          if (first < 0) first = e;
+          last = e;
+        }
+      if (before == 0) s_min = (int32_t)(key[first] >> 3);
+      if (before + cnt == emitted) s_max = (int32_t)(key[last] >> 3);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      counts[c] = emitted;
+      base[c] = s_min;
+      if (emitted > 0 && (int64_t)s_max - (int64_t)s_min >= ((int64_t)1 << 24)) atomicOr(overflow, 2);
+    }
+    return;
+  }
+  const int64_t out = cl_ptr[c] + before;
+  const uint32_t b0 = (uint32_t)base[c];
+  int r = 0;
+  for (int e = e0; e < e1; ++e)
+    if (val[e]) entries[out + r++] = (((uint32_t)(key[e] >> 3) - b0) << 8) | (uint32_t)val[e];
+}
+
 struct DevBuf {
   void *p = nullptr;
   ~DevBuf() {
@@ -588,5 +762,106 @@ extern "C" inch_status inch_hessian_assemble_pbc(int64_t n, const double *xyz, d
   const int64_t grid = (n * 64 + bs - 1) / bs;
   k_assemble<<<grid, bs, 0, st>>>(n, xyz, plus_i, rowptr, col, images, box_x, box_y, vals64, vals32, diag);
   HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                         double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
+                                         int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out,
+                                         void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && rowptr && col && cluster_atoms && cl_ptr && base && cell_xy > 0 && cell_z > 0,
+               "inch_anm_clusters: bad args");
+  INCH_REQUIRE(n < (int64_t)1 << 31, "inch_anm_clusters: n must fit int32");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t ncl = (n + kClusterAtoms - 1) / kClusterAtoms;
+  DevBuf ovf;
+  HIP_TRY(hipMalloc(&ovf.p, sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(ovf.p, 0, sizeof(int32_t), st));
+  auto check_overflow = [&](const char *pass) -> inch_status {
+    int32_t over = 0;
+    HIP_TRY(hipMemcpyAsync(&over, ovf.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (over & 1) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters (%s): a cluster's rows hold more than %d entries", pass, kMaxUnion);
+    if (over & 2) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters (%s): a cluster's neighbours span 2^24 columns or more", pass);
+    return INCH_OK;
+  };
+  if (entries) {  // fill pass on the layout of the count pass
+    int64_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (total > capacity) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters: need %lld > capacity %lld", (long long)total, (long long)capacity);
+    k_cl_union<true><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, cluster_atoms, cl_ptr, base, nullptr, entries,
+                                                    (int32_t *)ovf.p);
+    HIP_TRY(hipGetLastError());
+    inch_status s = check_overflow("fill");
+    if (s != INCH_OK) return s;
+    if (nnz_out) *nnz_out = total;
+    return INCH_OK;
+  }
+  // bounds
+  const int nb = 256;
+  DevBuf part;
+  HIP_TRY(hipMalloc(&part.p, nb * 6 * sizeof(double)));
+  k_minmax<<<nb, 256, 0, st>>>(n, xyz, (double *)part.p);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> hp(nb * 6);
+  HIP_TRY(hipMemcpyAsync(hp.data(), part.p, nb * 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  CGrid g;
+  double hi[3];
+  for (int a = 0; a < 3; ++a) {
+    g.lo[a] = INFINITY;
+    hi[a] = -INFINITY;
+    for (int b = 0; b < nb; ++b) {
+      g.lo[a] = std::min(g.lo[a], hp[b * 6 + a]);
+      hi[a] = std::max(hi[a], hp[b * 6 + 3 + a]);
+    }
+    if (!std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_anm_clusters: non-finite coordinates");
+  }
+  // cells: cell_xy in x and y; cell_z in z, coarsened until there are at most max(2n, 65536) cells
+  const double size[3] = {cell_xy, cell_xy, cell_z};
+  for (int a = 0; a < 3; ++a) {
+    g.inv[a] = 1.0 / size[a];
+    g.dim[a] = std::max<int64_t>(1, (int64_t)std::floor((hi[a] - g.lo[a]) * g.inv[a]) + 1);
+  }
+  const int64_t cap_cells = std::max<int64_t>(2 * n, 65536);
+  while (g.dim[0] * g.dim[1] * g.dim[2] > cap_cells && g.dim[2] > 1) {
+    g.inv[2] *= 0.5;
+    g.dim[2] = std::max<int64_t>(1, (int64_t)std::floor((hi[2] - g.lo[2]) * g.inv[2]) + 1);
+  }
+  const int64_t ncell = g.dim[0] * g.dim[1] * g.dim[2];
+  if (ncell > ((int64_t)1 << 30)) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters: %lld cells", (long long)ncell);
+  DevBuf cell_of, cnt, start, atoms, counts;
+  HIP_TRY(hipMalloc(&cell_of.p, n * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&cnt.p, (ncell + 1) * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&start.p, (ncell + 1) * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&atoms.p, n * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&counts.p, ncl * sizeof(int64_t)));
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
+  const int bs = 256;
+  const int64_t gb = (n + bs - 1) / bs;
+  k_cl_cell<<<gb, bs, 0, st>>>(n, xyz, g, (int32_t *)cell_of.p, (int32_t *)cnt.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_i32<<<1, 1024, 0, st>>>(ncell, (int32_t *)cnt.p, (int32_t *)start.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
+  k_cell_fill<<<gb, bs, 0, st>>>(n, (int32_t *)cell_of.p, (int32_t *)start.p, (int32_t *)cnt.p, (int32_t *)atoms.p);
+  HIP_TRY(hipGetLastError());
+  k_cell_isort<<<(unsigned)((ncell + bs - 1) / bs), bs, 0, st>>>(ncell, (int32_t *)start.p, (int32_t *)atoms.p);
+  HIP_TRY(hipGetLastError());
+  const int64_t slots = ncl * kClusterAtoms;
+  k_cl_atoms<<<(unsigned)((slots + bs - 1) / bs), bs, 0, st>>>(n, slots, (int32_t *)atoms.p, cluster_atoms);
+  HIP_TRY(hipGetLastError());
+  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, cluster_atoms, nullptr, base,
+                                                   (int64_t *)counts.p, nullptr, (int32_t *)ovf.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_i64<<<1, 1024, 0, st>>>(ncl, (int64_t *)counts.p, cl_ptr);
+  HIP_TRY(hipGetLastError());
+  inch_status s = check_overflow("count");
+  if (s != INCH_OK) return s;
+  int64_t total = 0;
+  HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (nnz_out) *nnz_out = total;
   return INCH_OK;
 }

@@ -213,8 +213,9 @@ __global__ void __launch_bounds__(256) k_matfree(int64_t n, const double *__rest
 // Single right-hand side with the same fused epilogue as k_bsr_spmv:
 //   out = s1 * (A x - c0 * x) - c2 * z (+ Q term); y = out; acc += mu * out.
 // G lanes per row; each lane handles U neighbours per iteration with independent loads
-// (memory-level parallelism for the dependent index -> coordinate/vector gathers).
-template <typename VT, int G, int U, typename CT>
+// (memory-level parallelism for the dependent index -> coordinate/vector gathers). XCD = true
+// gives each XCD a contiguous range of rows (xcd_block), so the band's gathers stay in its L2.
+template <typename VT, int G, int U, typename CT, bool XCD = false>
 __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__restrict__ xyz,
                                                       const int64_t *__restrict__ rowptr,
                                                       const CT *__restrict__ col,
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
                                                       VT *__restrict__ acc, double mu, const double *__restrict__ Q,
                                                       const double *__restrict__ qcoef, int mq) {
   const int lane = threadIdx.x & (G - 1);
-  const int64_t row = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / G;
+  const int64_t row = (xcd_block(XCD) * (int64_t)blockDim.x + threadIdx.x) / G;
   if (row >= n) return;
   const double pi0 = xyz[3 * row], pi1 = xyz[3 * row + 1], pi2 = xyz[3 * row + 2];
   const double vi0 = (double)x[3 * row], vi1 = (double)x[3 * row + 1], vi2 = (double)x[3 * row + 2];
@@ -270,6 +271,124 @@ __global__ void __launch_bounds__(256) k_matfree_epi(int64_t n, const double *__
     y[dof] = (VT)out;
     if (acc) acc[dof] = (VT)((double)acc[dof] + mu * out);
   }
+}
+
+// Cluster form of the matrix-free operator (layout from inch_anm_clusters): cluster c holds
+// 4 atoms that are close in space and lists the union of their neighbours once as
+// ((j - base[c]) << 8) | mask. One wave per cluster; the 64 lanes take 64 different list
+// entries, so one load instruction gathers 64 neighbours, each used for the 4 atoms (the
+// per-row kernel gathers every neighbour once per atom). The cluster's own coordinates and
+// vector entries are wave-uniform (scalar registers). Each lane keeps 4 x 3 partial sums,
+// reduced across the wave by halving exchanges in a fixed order. Per pair the arithmetic is
+// that of k_matfree_epi; only the order of the sum over neighbours differs.
+constexpr int kClAtoms = 4;
+
+template <typename VT, int U, bool XCD>
+__global__ void __launch_bounds__(64) k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl,
+                                                    const int32_t *__restrict__ cluster_atoms,
+                                                    const int64_t *__restrict__ cl_ptr,
+                                                    const int32_t *__restrict__ base,
+                                                    const uint32_t *__restrict__ entries, double plus_i,
+                                                    const VT *__restrict__ x, VT *__restrict__ y, double s1,
+                                                    double c0, const VT *__restrict__ z, double c2,
+                                                    VT *__restrict__ acc, double mu, const double *__restrict__ Q,
+                                                    const double *__restrict__ qcoef, int mq) {
+  const int64_t c = xcd_block(XCD);
+  if (c >= ncl) return;
+  const int lane = threadIdx.x;
+  double pi[kClAtoms][3], vi[kClAtoms][3];
+  double v[16];  // 4 atoms x 3 partial sums, padded to 16 for the halving reduction
+#pragma unroll
+  for (int t = 0; t < kClAtoms; ++t) {
+    const int32_t at = cluster_atoms[kClAtoms * c + t];
+    const int64_t i = at >= 0 ? at : 0;  // a padding slot has no list bits
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      pi[t][k] = xyz[3 * i + k];
+      vi[t][k] = (double)x[3 * i + k];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) v[q] = 0.0;
+  // the epilogue's own loads depend only on the cluster's atoms: issue them before the list
+  const int idx = lane >> 2;
+  const bool writer = (lane & 3) == 0 && idx < 3 * kClAtoms;
+  const int32_t at_w = writer ? cluster_atoms[kClAtoms * c + idx / 3] : -1;
+  const int64_t dof = 3 * (int64_t)(at_w >= 0 ? at_w : 0) + idx % 3;
+  double xv = 0.0, zv = 0.0, av = 0.0;
+  if (at_w >= 0) {
+    xv = (double)x[dof];
+    if (z) zv = (double)z[dof];
+    if (acc) av = (double)acc[dof];
+  }
+  const int64_t e1 = cl_ptr[c + 1];
+  const int64_t cb = base[c];
+  const int64_t estart = cl_ptr[c];
+  // the next iteration's list entries are loaded while this iteration's gathers are in flight
+  uint32_t wn[U];
+#pragma unroll
+  for (int q = 0; q < U; ++q) {
+    const int64_t e = estart + q * 64 + lane;
+    wn[q] = e < e1 ? entries[e] : 0u;  // mask 0: no term
+  }
+  for (int64_t e0 = estart; e0 < e1; e0 += 64 * U) {
+    uint32_t w[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      w[q] = wn[q];
+      const int64_t e = e0 + 64 * U + q * 64 + lane;
+      wn[q] = e < e1 ? entries[e] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int64_t j = cb + (int64_t)(w[q] >> 8);
+      const double pj0 = xyz[3 * j], pj1 = xyz[3 * j + 1], pj2 = xyz[3 * j + 2];
+      const double xj0 = (double)x[3 * j], xj1 = (double)x[3 * j + 1], xj2 = (double)x[3 * j + 2];
+#pragma unroll
+      for (int t = 0; t < kClAtoms; ++t) {
+        const double d0 = pj0 - pi[t][0], d1 = pj1 - pi[t][1], d2 = pj2 - pi[t][2];
+        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+        const double w0 = vi[t][0] - xj0, w1 = vi[t][1] - xj1, w2 = vi[t][2] - xj2;
+        // r2 == 0 (coincident atoms) is masked as in the reference
+        const bool on = ((w[q] >> t) & 1u) && r2 > 0.0;
+        const double s = on ? (d0 * w0 + d1 * w1 + d2 * w2) * recip_r2(r2) : 0.0;
+        v[3 * t] += d0 * s;
+        v[3 * t + 1] += d1 * s;
+        v[3 * t + 2] += d2 * s;
+      }
+    }
+  }
+  // halving exchanges: at offset o the lane keeps one half of its values (upper half when
+  // lane & o) summed with its partner's copy of that half; after o = 32..4, lanes sharing
+  // bits 2..5 hold partial sums of value (lane >> 2); two full exchanges finish the sums.
+  // Both halves are exchanged and the choice is made on the sums: a choice between two
+  // array elements would become a dynamically indexed register array.
+#pragma unroll
+  for (int o = 32, m = 8; o >= 4; o >>= 1, m >>= 1) {
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int q = 0; q < m; ++q) {
+      const double lo = v[q] + __shfl_xor(v[q], o, 64);
+      const double hi = v[q + m] + __shfl_xor(v[q + m], o, 64);
+      v[q] = up ? hi : lo;
+    }
+  }
+  double sum = v[0];
+  sum += __shfl_xor(sum, 1, 64);
+  sum += __shfl_xor(sum, 2, 64);
+  if (at_w < 0) return;
+  double out = sum + plus_i * xv;
+  if (c0 != 0.0) out -= c0 * xv;
+  out *= s1;
+  if (z) out -= c2 * zv;
+  if (qcoef) {
+    double qs = 0.0;
+    const int64_t nd = 3 * n;
+    for (int m = 0; m < mq; ++m) qs += qcoef[m] * Q[m * nd + dof];
+    out += qs;
+  }
+  y[dof] = (VT)out;
+  if (acc) acc[dof] = (VT)(av + mu * out);
 }
 
 // Periodic images in x and y: the entry (i, j) carries a 9-bit mask of the images
@@ -577,13 +696,13 @@ inch_status launch_spmm(int64_t n, const int64_t *rowptr, const int32_t *col, co
   return INCH_OK;
 }
 
-template <typename VT, int G, int U, typename CT>
+template <typename VT, int G, int U, typename CT, bool XCD>
 void launch_matfree(int64_t n, const double *xyz, const int64_t *rowptr, const CT *col, const int32_t *rowbase,
                     double plus_i, const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc,
                     double mu, const double *Q, const double *qcoef, int m_q, hipStream_t st) {
   const int bs = 256;
   const int64_t grid = (n * G + bs - 1) / bs;
-  k_matfree_epi<VT, G, U, CT><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, rowbase, plus_i, (const VT *)x, (VT *)y, s1,
+  k_matfree_epi<VT, G, U, CT, XCD><<<grid, bs, 0, st>>>(n, xyz, rowptr, col, rowbase, plus_i, (const VT *)x, (VT *)y, s1,
                                                    c0, (const VT *)z, c2, (VT *)acc, mu, Q, qcoef, m_q);
 }
 
@@ -594,8 +713,12 @@ bool dispatch_matfree(int code, int64_t n, const double *xyz, const int64_t *row
                       hipStream_t st) {
 #define INCH_MF(GG, UU)                                                                                        \
   case GG * 10 + UU:                                                                                           \
-    launch_matfree<VT, GG, UU, CT>(n, xyz, rowptr, col, rowbase, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, qcoef, \
-                                   m_q, st);                                                                   \
+    launch_matfree<VT, GG, UU, CT, false>(n, xyz, rowptr, col, rowbase, plus_i, x, y, s1, c0, z, c2, acc, mu, Q, \
+                                          qcoef, m_q, st);                                                     \
+    return true;                                                                                               \
+  case 2000 + GG * 10 + UU:                                                                                    \
+    launch_matfree<VT, GG, UU, CT, true>(n, xyz, rowptr, col, rowbase, plus_i, x, y, s1, c0, z, c2, acc, mu, Q,  \
+                                         qcoef, m_q, st);                                                      \
     return true;
   switch (code) {
     INCH_MF(4, 1)
@@ -610,6 +733,40 @@ bool dispatch_matfree(int code, int64_t n, const double *xyz, const int64_t *row
       return false;
   }
 #undef INCH_MF
+}
+
+template <typename VT, int U, bool XCD>
+void launch_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const int32_t *catoms, const int64_t *cl_ptr,
+                       const int32_t *base, const uint32_t *entries, double plus_i, const void *x, void *y, double s1,
+                       double c0, const void *z, double c2, void *acc, double mu, const double *Q,
+                       const double *qcoef, int m_q, hipStream_t st) {
+  k_matfree_cl<VT, U, XCD><<<(unsigned)ncl, 64, 0, st>>>(n, xyz, ncl, catoms, cl_ptr, base, entries, plus_i,
+                                                         (const VT *)x, (VT *)y, s1, c0, (const VT *)z, c2, (VT *)acc,
+                                                         mu, Q, qcoef, m_q);
+}
+
+template <typename VT>
+bool dispatch_matfree_cl(int code, int64_t n, const double *xyz, int64_t ncl, const int32_t *catoms,
+                         const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double plus_i,
+                         const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc, double mu,
+                         const double *Q, const double *qcoef, int m_q, hipStream_t st) {
+#define INCH_CL(UU)                                                                                              \
+  case UU:                                                                                                       \
+    launch_matfree_cl<VT, UU, false>(n, xyz, ncl, catoms, cl_ptr, base, entries, plus_i, x, y, s1, c0, z, c2, acc, \
+                                     mu, Q, qcoef, m_q, st);                                                     \
+    return true;                                                                                                 \
+  case 2000 + UU:                                                                                                \
+    launch_matfree_cl<VT, UU, true>(n, xyz, ncl, catoms, cl_ptr, base, entries, plus_i, x, y, s1, c0, z, c2, acc,  \
+                                    mu, Q, qcoef, m_q, st);                                                      \
+    return true;
+  switch (code) {
+    INCH_CL(1)
+    INCH_CL(2)
+    INCH_CL(4)
+    default:
+      return false;
+  }
+#undef INCH_CL
 }
 
 }  // namespace
@@ -726,7 +883,7 @@ inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *r
   else
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: bad dtype");
   if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv: config %d is not lanes*10+unroll with lanes in "
-                                       "{4,8,16,32} and unroll in {1,2}", code);
+                                       "{4,8,16,32} and unroll in {1,2}, plus 2000 for XCD-contiguous rows", code);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
@@ -776,7 +933,33 @@ inch_status inch_anm_matfree_spmv16(int64_t n, const double *xyz, const int64_t 
   else
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv16: bad dtype");
   if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_spmv16: config %d is not lanes*10+unroll with lanes in "
-                                       "{4,8,16,32} and unroll in {1,2}", code);
+                                       "{4,8,16,32} and unroll in {1,2}, plus 2000 for XCD-contiguous rows", code);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
+                                const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double plus_i,
+                                int32_t vdt, const void *x, void *y, double s1, double c0, const void *z, double c2,
+                                void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
+                                int32_t config, void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && cluster_atoms && cl_ptr && base && x && y && x != y,
+               "inch_anm_matfree_cl: bad args");
+  INCH_REQUIRE(ncl == (n + kClAtoms - 1) / kClAtoms, "inch_anm_matfree_cl: ncl must be ceil(n / 4)");
+  INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_cl: qcoef needs Q");
+  hipStream_t st = (hipStream_t)stream;
+  const int code = config > 0 ? config : 2002;
+  bool ok;
+  if (vdt == INCH_F64)
+    ok = dispatch_matfree_cl<double>(code, n, xyz, ncl, cluster_atoms, cl_ptr, base, entries, plus_i, x, y, s1, c0,
+                                     z, c2, acc, mu, Q, qcoef, m_q, st);
+  else if (vdt == INCH_F32)
+    ok = dispatch_matfree_cl<float>(code, n, xyz, ncl, cluster_atoms, cl_ptr, base, entries, plus_i, x, y, s1, c0,
+                                    z, c2, acc, mu, Q, qcoef, m_q, st);
+  else
+    INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: bad dtype");
+  if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: config %d is not an unroll in {1,2,4}, plus 2000 for "
+                                       "XCD-contiguous clusters", code);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

@@ -78,6 +78,21 @@ class BlockHessian:
                 self._col16 = None
         return self._col16
 
+    def clusters(self, cell_xy=None, cell_z=None):
+        """Cluster layout of the matrix-free operator (ops.anm_clusters; built once), or None for
+        periodic matrices and when the layout does not fit. Cells default to 0.5 rc across and
+        0.25 rc along z, which for protein densities gives clusters of 4 atoms a union of about
+        1.6 times one atom's neighbours."""
+        if "_clusters" not in self.__dict__:
+            self._clusters = None
+            if self.images is None:
+                try:
+                    self._clusters = ops.anm_clusters(self.xyz, self.rowptr, self.col,
+                                                      cell_xy or 0.5 * self.rc, cell_z or 0.25 * self.rc)
+                except ops.InchingError:
+                    self._clusters = None
+        return self._clusters
+
     def ensure(self, precision):
         """Make sure the values of ``precision`` exist (assembled on demand)."""
         if (precision == "f64" and self.vals64 is None) or (precision == "f32" and self.vals32 is None):
@@ -112,15 +127,23 @@ class BlockHessian:
 
     def matfree_spmv(self, x, y, config=0, **epi):
         """Matrix-free float64 product with the fused epilogue (``config``: autotuner code
-        lanes*10+unroll, plus 1000 for 16-bit column offsets, 0 = default; periodic matrices
-        use their own kernel)."""
+        lanes*10+unroll, plus 1000 for 16-bit column offsets, 2000 for XCD-contiguous rows and
+        4000 for the cluster layout, 0 = default; periodic matrices use their own kernel)."""
         if self.images is not None:
             ops.matfree_pbc(self.xyz, self.rowptr, self.col, self.images, self.box, self.plus_i, x, y, **epi)
-        elif config >= 1000 and self.col16() is not None:
-            ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config - 1000,
-                             col16=self.col16(), **epi)
-        else:
-            ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=config % 1000, **epi)
+            return
+        shape, flags = config % 1000, config // 1000
+        abi = shape + (2000 if flags & 2 else 0) if shape else 0
+        if flags & 4:
+            cl = self.clusters()
+            if cl is None:
+                raise ValueError("config %d asks for the cluster layout, which this matrix cannot use" % config)
+            ops.matfree_cl(self.xyz, cl, self.plus_i, x, y, config=abi, **epi)
+            return
+        c16 = self.col16() if flags & 1 else None
+        if flags & 1 and c16 is None:
+            raise ValueError("config %d asks for 16-bit column offsets, which this matrix cannot use" % config)
+        ops.matfree_spmv(self.xyz, self.rowptr, self.col, self.plus_i, x, y, config=abi, col16=c16, **epi)
 
     def matfree(self, X):
         """FP64 (H + PlusI I) X recomputed from coordinates (X [3n] or [3n, r])."""

@@ -213,6 +213,59 @@ def bsr_col16(n_atoms, rowptr, col):
     return rowbase, col16[:col.numel()]
 
 
+CLUSTER_ATOMS = 4   # atoms per cluster of the cluster layout (inch_anm_clusters)
+
+
+def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None):
+    """Cluster layout of the matrix-free operator (inch_anm_clusters):
+    (cluster_atoms int32 [ceil(n/4)*4], cl_ptr int64 [ncl+1], base int32 [ncl],
+    entries int32 [nnz] holding the uint32 words ((j - base) << 8) | mask).
+    Raises InchingError (INCH_ERR_CAPACITY) when the layout does not fit the matrix."""
+    require_gpu()
+    _check_dev(xyz)
+    n = xyz.shape[0]
+    if rowptr.numel() != n + 1:
+        raise ValueError("rowptr must have n + 1 entries")
+    ncl = (n + CLUSTER_ATOMS - 1) // CLUSTER_ATOMS
+    dev = xyz.device
+    atoms = torch.empty(ncl * CLUSTER_ATOMS, dtype=torch.int32, device=dev)
+    ptr = torch.empty(ncl + 1, dtype=torch.int64, device=dev)
+    base = torch.empty(ncl, dtype=torch.int32, device=dev)
+    nnz = C.c_int64(0)
+    call("inch_anm_clusters", n, P(xyz), P(rowptr), P(col), float(cell_xy), float(cell_z), P(atoms), P(ptr), P(base),
+         None, 0, C.byref(nnz), stream_ptr(stream))
+    entries = torch.empty(max(1, nnz.value), dtype=torch.int32, device=dev)
+    call("inch_anm_clusters", n, P(xyz), P(rowptr), P(col), float(cell_xy), float(cell_z), P(atoms), P(ptr), P(base),
+         P(entries), nnz.value, C.byref(nnz), stream_ptr(stream))
+    return atoms, ptr, base, entries[:nnz.value]
+
+
+def matfree_cl(xyz, clusters, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None, qcoef=None,
+               config=0, stream=None):
+    """Matrix-free ANM product with the fused epilogue on the cluster layout ``clusters``
+    (the tuple of anm_clusters)."""
+    n = xyz.shape[0]
+    nd = 3 * n
+    for t in (x, y, z, acc):
+        if t is not None and t.numel() != nd:
+            raise ValueError("vector length mismatch")
+    vd = dt(x)
+    for t in (y, z, acc):
+        if t is not None and dt(t) != vd:
+            raise ValueError("all vectors must share a dtype")
+    mq = 0
+    if qcoef is not None:
+        if Q is None or Q.dtype != torch.float64 or Q.shape[1] != nd:
+            raise ValueError("Q must be float64 [m, 3n]")
+        mq = Q.shape[0]
+    atoms, ptr, base, entries = clusters
+    ncl = (n + CLUSTER_ATOMS - 1) // CLUSTER_ATOMS
+    if atoms.numel() != CLUSTER_ATOMS * ncl or ptr.numel() != ncl + 1 or base.numel() != ncl:
+        raise ValueError("cluster layout does not match n")
+    call("inch_anm_matfree_cl", n, P(xyz), ncl, P(atoms), P(ptr), P(base), P(entries), float(plus_i), vd, P(x), P(y),
+         float(s1), float(c0), P(z), float(c2), P(acc), float(mu), P(Q), P(qcoef), mq, int(config), stream_ptr(stream))
+
+
 def bsr_spmv(n_atoms, rowptr, col, vals, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None,
              qcoef=None, lanes=16, stream=None, col16=None):
     """y = s1 (A x - c0 x) - c2 z (+ sum_m qcoef[m] Q[m]); acc += mu y (all terms optional).

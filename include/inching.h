@@ -150,7 +150,7 @@ inch_status inch_anm_matfree_apply(int64_t n, const double *xyz, const int64_t *
 /* Matrix-free ANM product with the fused epilogue of inch_bsr_spmv (vectors float32
  * or float64 per vdt; geometry and accumulation in float64). config = lanes*10 + unroll
  * (lanes per row 4/8/16/32, neighbours per lane iteration 1/2; 0 = 161), the autotuned
- * launch shape. */
+ * launch shape; + 2000 maps each XCD's workgroups to one contiguous range of rows. */
 inch_status inch_anm_matfree_spmv(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
                                   double plus_i, int32_t vdt, const void *x, void *y, double s1, double c0,
                                   const void *z, double c2, void *acc, double mu, const double *Q,
@@ -198,6 +198,31 @@ inch_status inch_bsr_col16(int64_t n, const int64_t *rowptr, const int32_t *col,
 
 /* dst = src (bytes, multiple of 16): streaming copy for measuring achievable HBM bandwidth. */
 inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *stream);
+
+/* Cluster layout of the matrix-free operator: another layout of the neighbour CSR that
+ * inch_anm_matfree_spmv reads (same reference interfaces). The atoms are grouped into
+ * ncl = ceil(n / 4) clusters that are compact in space: cells of cell_xy x cell_xy x cell_z
+ * (z cells coarsened until there are at most max(2n, 65536) cells), numbered by x, y column
+ * then z, atoms in a cell by index; cluster c holds atoms [4c, 4c + 4) of that order,
+ * cluster_atoms int32[ncl * 4] (-1 pads the last cluster). Each cluster lists the union of its
+ * atoms' rows once: entries uint32 = ((j - base[c]) << 8) | mask, ascending j, mask bit t set
+ * when j is a neighbour of atom t of the cluster (j itself excluded); cl_ptr int64[ncl + 1],
+ * base int32[ncl]. Two passes, as inch_neighbors_celllist: with entries == NULL it writes
+ * cluster_atoms, cl_ptr, base and *nnz_out; the call with entries (capacity >= nnz) and the
+ * arrays of the first pass writes the entries. INCH_ERR_CAPACITY when a cluster's rows hold
+ * more than 4096 entries or its neighbours span 2^24 columns or more (syncs). */
+inch_status inch_anm_clusters(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                              double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
+                              int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out, void *stream);
+/* inch_anm_matfree_spmv on the cluster layout: the same product and fused epilogue, the
+ * neighbour's coordinates and vector entries loaded once per cluster (one wave per cluster,
+ * lanes over its list). config = list entries per lane iteration (1, 2 or 4; 0 = 2002),
+ * + 2000 maps each XCD's workgroups to one contiguous range of clusters. */
+inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
+                                const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double plus_i,
+                                int32_t vdt, const void *x, void *y, double s1, double c0, const void *z, double c2,
+                                void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
+                                int32_t config, void *stream);
 
 /* h[j] = sum_i V[j*ld + i] * u[i] for j < J (float64, deterministic two-phase
  * reduction). work: float64 scratch of inch_multidot_work_size(J) elements. */

@@ -370,10 +370,108 @@ def test_matfree_launch_configs_agree(mat):
         out = 0.3 * (Bs @ x - 2.0 * x) - z
         assert np.allclose(y.cpu().numpy(), out, rtol=1e-12, atol=1e-12), cfg
         assert np.allclose(acc.cpu().numpy(), acc0 + 0.5 * out, rtol=1e-12, atol=1e-12)
-        # 16-bit column offsets (config + 1000): bitwise the same result
-        y16 = torch.empty_like(y)
-        acc16 = _dev(acc0)
-        A.matfree_spmv(_dev(x), y16, config=1000 + cfg, s1=0.3, c0=2.0, z=_dev(z), c2=1.0, acc=acc16, mu=0.5)
-        assert torch.equal(y16, y) and torch.equal(acc16, acc), cfg
+        # 16-bit column offsets (+1000), XCD-contiguous rows (+2000), both (+3000): bitwise the same
+        for flag in (1000, 2000, 3000):
+            y16 = torch.empty_like(y)
+            acc16 = _dev(acc0)
+            A.matfree_spmv(_dev(x), y16, config=flag + cfg, s1=0.3, c0=2.0, z=_dev(z), c2=1.0, acc=acc16, mu=0.5)
+            assert torch.equal(y16, y) and torch.equal(acc16, acc), (flag, cfg)
         if ref is None:
             ref = y.cpu().numpy()
+
+
+def _decode_clusters(cl):
+    """Host decode of the cluster layout: (cluster atoms [ncl, 4], list of (j array, mask array))."""
+    atoms, ptr, base, entries = (t.cpu().numpy() for t in cl)
+    w = entries.view(np.uint32)
+    out = []
+    for c in range(base.size):
+        e = w[ptr[c]:ptr[c + 1]]
+        out.append((base[c] + (e >> 8).astype(np.int64), (e & 0xFF).astype(np.int64)))
+    return atoms.reshape(-1, 4), out
+
+
+def test_cluster_layout_reproduces_csr(mat):
+    """Every neighbour pair (i, j != i) of the CSR appears exactly once in the cluster lists,
+    and nothing else does (bit-exact structure); clusters partition the atoms; deterministic."""
+    from inching_amd import ops
+    A, _ = mat
+    _print("case: cluster layout n=%d" % A.n_atoms)
+    cl = A.clusters()
+    assert cl is not None
+    catoms, lists = _decode_clusters(cl)
+    n = A.n_atoms
+    flat = catoms.reshape(-1)
+    assert np.array_equal(np.sort(flat[flat >= 0]), np.arange(n))
+    assert np.all(flat[:n] >= 0) and np.all(flat[n:] == -1)
+    pairs = []
+    for c, (j, m) in enumerate(lists):
+        assert np.all(np.diff(j) > 0) and np.all(m > 0) and np.all(m < 16)
+        if j.size:
+            assert j[0] == cl[2][c].item()
+        for t in range(4):
+            sel = (m >> t) & 1 == 1
+            if sel.any():
+                assert catoms[c, t] >= 0
+                pairs.append(catoms[c, t] * np.int64(n) + j[sel])
+    got = np.sort(np.concatenate(pairs))
+    rp, col = A.rowptr.cpu().numpy(), A.col.cpu().numpy().astype(np.int64)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = rows != col
+    ref = np.sort(rows[off] * np.int64(n) + col[off])
+    assert np.array_equal(got, ref)
+    again = ops.anm_clusters(A.xyz, A.rowptr, A.col, 0.5 * A.rc, 0.25 * A.rc)
+    for a, b in zip(cl, again):
+        assert torch.equal(a, b)
+
+
+def test_cluster_product_matches_oracle(mat, glob2k):
+    """All cluster launch shapes (plain and XCD-contiguous) against the oracle's float64 block
+    Hessian, with the Chebyshev epilogue and the rigid-body term; float32 vectors too."""
+    from inching_amd.autotune import CLUSTER_CONFIGS
+    from inching_amd.hessian import rigid_body_basis
+    A, Bs = mat
+    X = A.xyz.cpu().numpy()
+    rp, col, vals = OH.block_hessian(X, 8.0, 1.0)
+    Bo = OH.block_to_scipy(rp, col, vals, A.n_atoms)
+    Q = rigid_body_basis(A.xyz)
+    rng = np.random.default_rng(31)
+    x, z, acc0 = (rng.standard_normal(A.shape[0]) for _ in range(3))
+    qc = rng.standard_normal(Q.shape[0])
+    Qh = Q.cpu().numpy()
+    out = 0.3 * (Bo @ x - 2.0 * x) - z + qc @ Qh
+    scale = np.abs(out).max()
+    for flag in (4000, 6000):
+        for cfg in CLUSTER_CONFIGS:
+            _print("case: cluster product config %d" % (flag + cfg))
+            y = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
+            acc = _dev(acc0)
+            A.matfree_spmv(_dev(x), y, config=flag + cfg, s1=0.3, c0=2.0, z=_dev(z), c2=1.0, acc=acc, mu=0.5, Q=Q,
+                           qcoef=_dev(qc))
+            assert np.abs(y.cpu().numpy() - out).max() <= 1e-12 * scale, flag + cfg
+            assert np.abs(acc.cpu().numpy() - (acc0 + 0.5 * out)).max() <= 1e-12 * max(1.0, scale)
+    # plain product (no epilogue) and float32 vectors
+    y = torch.empty(A.shape[0], dtype=torch.float64, device="cuda")
+    A.matfree_spmv(_dev(x), y, config=6002)
+    ref = Bo @ x
+    assert np.abs(y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    y32 = torch.empty(A.shape[0], dtype=torch.float32, device="cuda")
+    A.matfree_spmv(_dev(x.astype(np.float32)), y32, config=6002)
+    ref32 = Bo @ x.astype(np.float32).astype(np.float64)
+    assert np.linalg.norm(y32.cpu().numpy() - ref32) <= 1e-6 * np.linalg.norm(ref32)
+
+
+def test_cluster_capacity_falls_back():
+    """A cluster whose rows hold more than 4096 entries makes the layout unavailable (a host
+    error, no kernel fault); the operator then keeps the per-row kernels."""
+    from inching_amd import ops
+    from inching_amd.autotune import best_config
+    from inching_amd.hessian import build_hessian
+    _print("case: cluster capacity, 1500 atoms within one cutoff")
+    rng = np.random.default_rng(5)
+    X = rng.uniform(0.0, 12.0, size=(1500, 3))
+    A = build_hessian(X, 30.0, plus_i=1.0, reorder=True, precision=())
+    with pytest.raises(ops.InchingError):
+        ops.anm_clusters(A.xyz, A.rowptr, A.col, 0.5 * A.rc, 0.25 * A.rc)
+    assert A.clusters() is None
+    assert best_config(A, "mf64") // 1000 & 4 == 0
