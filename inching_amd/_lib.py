@@ -45,6 +45,7 @@ SIGNATURES = {
     "inch_bsr_col16": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "inch_stream_copy": (_STATUS, [_i64, _vp, _vp, _vp]),
     "inch_anm_clusters": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "inch_anm_matfree_cl_apply": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _i32, _vp]),
     "inch_anm_matfree_cl": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _vp, _dbl, _dbl, _vp,
                                       _dbl, _vp, _dbl, _vp, _vp, _i32, _i32, _vp]),
     "inch_multidot_work_size": (_i64, [_i32]),

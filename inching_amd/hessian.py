@@ -150,6 +150,8 @@ class BlockHessian:
         Y = torch.empty_like(X)
         if self.images is not None:
             ops.matfree_pbc(self.xyz, self.rowptr, self.col, self.images, self.box, self.plus_i, X.contiguous(), Y)
+        elif self.clusters() is not None:
+            ops.matfree_cl_apply(self.xyz, self.clusters(), self.plus_i, X.contiguous(), Y)
         else:
             ops.matfree_apply(self.xyz, self.rowptr, self.col, self.plus_i, X.contiguous(), Y)
         return Y

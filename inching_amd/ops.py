@@ -240,6 +240,23 @@ def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None):
     return atoms, ptr, base, entries[:nnz.value]
 
 
+def matfree_cl_apply(xyz, clusters, plus_i, X, Y, stream=None):
+    """Y = (H + plus_i I) X on the cluster layout, float64 X [3n] or [3n, r] row-major."""
+    n = xyz.shape[0]
+    X2 = X.reshape(3 * n, -1)
+    Y2 = Y.reshape(3 * n, -1)
+    if X2.dtype != torch.float64 or Y2.dtype != torch.float64 or not X2.is_contiguous() or not Y2.is_contiguous():
+        raise ValueError("X and Y must be contiguous float64 [3n, r]")
+    if Y2.shape != X2.shape:
+        raise ValueError("X and Y shapes differ")
+    atoms, ptr, base, entries = clusters
+    ncl = (n + CLUSTER_ATOMS - 1) // CLUSTER_ATOMS
+    if atoms.numel() != CLUSTER_ATOMS * ncl or ptr.numel() != ncl + 1 or base.numel() != ncl:
+        raise ValueError("cluster layout does not match n")
+    call("inch_anm_matfree_cl_apply", n, P(xyz), ncl, P(atoms), P(ptr), P(base), P(entries), float(plus_i), P(X2),
+         P(Y2), X2.shape[1], stream_ptr(stream))
+
+
 def matfree_cl(xyz, clusters, plus_i, x, y, s1=1.0, c0=0.0, z=None, c2=0.0, acc=None, mu=0.0, Q=None, qcoef=None,
                config=0, stream=None):
     """Matrix-free ANM product with the fused epilogue on the cluster layout ``clusters``

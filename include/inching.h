@@ -223,6 +223,11 @@ inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const
                                 int32_t vdt, const void *x, void *y, double s1, double c0, const void *z, double c2,
                                 void *acc, double mu, const double *Q, const double *qcoef, int32_t m_q,
                                 int32_t config, void *stream);
+/* inch_anm_matfree_apply on the cluster layout: Y = (H + plus_i I) X for nrhs row-major
+ * right-hand sides (X, Y float64 [3n, nrhs]). */
+inch_status inch_anm_matfree_cl_apply(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
+                                      const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries,
+                                      double plus_i, const double *x, double *y, int32_t nrhs, void *stream);
 
 /* h[j] = sum_i V[j*ld + i] * u[i] for j < J (float64, deterministic two-phase
  * reduction). work: float64 scratch of inch_multidot_work_size(J) elements. */

@@ -475,3 +475,24 @@ def test_cluster_capacity_falls_back():
         ops.anm_clusters(A.xyz, A.rowptr, A.col, 0.5 * A.rc, 0.25 * A.rc)
     assert A.clusters() is None
     assert best_config(A, "mf64") // 1000 & 4 == 0
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 7, 64])
+def test_cluster_apply_multi_rhs(mat, nrhs):
+    """Several right-hand sides on the cluster layout against the oracle's block Hessian and
+    the per-row multi-RHS kernel."""
+    from inching_amd import ops
+    A, _ = mat
+    _print("case: cluster apply nrhs=%d" % nrhs)
+    X = A.xyz.cpu().numpy()
+    rp, col, vals = OH.block_hessian(X, 8.0, 1.0)
+    Bo = OH.block_to_scipy(rp, col, vals, A.n_atoms)
+    rng = np.random.default_rng(40 + nrhs)
+    V = rng.standard_normal((A.shape[0], nrhs))
+    Y = torch.empty((A.shape[0], nrhs), dtype=torch.float64, device="cuda")
+    ops.matfree_cl_apply(A.xyz, A.clusters(), A.plus_i, _dev(V), Y)
+    ref = Bo @ V
+    assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
+    Yr = torch.empty_like(Y)
+    ops.matfree_apply(A.xyz, A.rowptr, A.col, A.plus_i, _dev(V), Yr)
+    assert torch.allclose(Y, Yr, rtol=0, atol=1e-12 * float(np.abs(ref).max()))
