@@ -772,6 +772,8 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   INCH_REQUIRE(n > 0 && xyz && rowptr && col && cluster_atoms && cl_ptr && base && cell_xy > 0 && cell_z > 0,
                "inch_anm_clusters: bad args");
   INCH_REQUIRE(n < (int64_t)1 << 31, "inch_anm_clusters: n must fit int32");
+  // the cluster kernels address coordinates and vectors with 32-bit byte offsets
+  if (24 * n > (int64_t)UINT32_MAX) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters: n = %lld exceeds the 32-bit offset range (%lld atoms)", (long long)n, (long long)(UINT32_MAX / 24));
   hipStream_t st = (hipStream_t)stream;
   const int64_t ncl = (n + kClusterAtoms - 1) / kClusterAtoms;
   DevBuf ovf;

@@ -305,16 +305,37 @@ __device__ __forceinline__ double swap_sum(double a, double b) {
   }
 }
 
+// Raw buffer loads with 32-bit byte offsets from a descriptor built per array (word 3 of
+// the gfx9-family descriptor is 0x00020000): no 64-bit address arithmetic per gather, and
+// a load at or past num_records bytes returns 0 (hardware range check).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// three consecutive VT at byte offset off, widened to double
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, uint32_t off, const double *, double (&o)[3]) {
+  const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+  const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off + 16), 0, 0);
+  o[0] = __hiloint2double((int)a[1], (int)a[0]);
+  o[1] = __hiloint2double((int)a[3], (int)a[2]);
+  o[2] = __hiloint2double((int)b[1], (int)b[0]);
+}
+__device__ __forceinline__ void load3(__amdgpu_buffer_rsrc_t r, uint32_t off, const float *, double (&o)[3]) {
+  const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+  const auto b = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 8), 0, 0);
+  o[0] = (double)__uint_as_float(a[0]);
+  o[1] = (double)__uint_as_float(a[1]);
+  o[2] = (double)__uint_as_float(b);
+}
+
+// The float64 single-entry form fits 80 VGPRs (6 waves per SIMD) without spilling.
 template <typename VT, int U, bool XCD>
-__global__ void __launch_bounds__(64) k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl,
-                                                    const int32_t *__restrict__ cluster_atoms,
-                                                    const int64_t *__restrict__ cl_ptr,
-                                                    const int32_t *__restrict__ base,
-                                                    const uint32_t *__restrict__ entries, double plus_i,
-                                                    const VT *__restrict__ x, VT *__restrict__ y, double s1,
-                                                    double c0, const VT *__restrict__ z, double c2,
-                                                    VT *__restrict__ acc, double mu, const double *__restrict__ Q,
-                                                    const double *__restrict__ qcoef, int mq) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(U == 1 && sizeof(VT) == 8 ? 6 : 1)))
+k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32_t *__restrict__ cluster_atoms,
+             const int64_t *__restrict__ cl_ptr, const int32_t *__restrict__ base,
+             const uint32_t *__restrict__ entries, double plus_i, const VT *__restrict__ x, VT *__restrict__ y,
+             double s1, double c0, const VT *__restrict__ z, double c2, VT *__restrict__ acc, double mu,
+             const double *__restrict__ Q, const double *__restrict__ qcoef, int mq) {
   const int64_t c = xcd_block(XCD);
   if (c >= ncl) return;
   const int lane = threadIdx.x;
@@ -343,33 +364,31 @@ __global__ void __launch_bounds__(64) k_matfree_cl(int64_t n, const double *__re
     if (z) zv = (double)z[dof];
     if (acc) av = (double)acc[dof];
   }
-  const int64_t e1 = cl_ptr[c + 1];
-  const int64_t cb = base[c];
   const int64_t estart = cl_ptr[c];
-  // the next iteration's list entries are loaded while this iteration's gathers are in
-  // flight; the loads are unconditional (index clamped to the list's last entry, value
-  // masked afterwards) so that waiting for them never waits for the newer ones
-  const int64_t elast = e1 > estart ? e1 - 1 : estart;
+  const uint32_t len = (uint32_t)(cl_ptr[c + 1] - estart);  // < 4096 (inch_anm_clusters)
+  const uint32_t cb = (uint32_t)base[c];
+  // 32-bit offsets: the cluster's own list (entries past it read as 0 = no term), and the
+  // coordinate / vector rows (24 n and 3 n sizeof(VT) bytes < 2^32, inch_anm_clusters)
+  const __amdgpu_buffer_rsrc_t rl = buf_rsrc(entries + estart, 4u * len);
+  const __amdgpu_buffer_rsrc_t rp = buf_rsrc(xyz, (uint32_t)(24 * n));
+  const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x, (uint32_t)(3 * sizeof(VT) * n));
+  // the next iteration's list entries are loaded while this iteration's gathers are in flight
   uint32_t wn[U];
-  if (e1 > estart) {
 #pragma unroll
-    for (int q = 0; q < U; ++q) wn[q] = entries[min(estart + q * 64 + lane, elast)];
-  }
-  for (int64_t e0 = estart; e0 < e1; e0 += 64 * U) {
+  for (int q = 0; q < U; ++q) wn[q] = __builtin_amdgcn_raw_buffer_load_b32(rl, (int)(4u * (q * 64 + lane)), 0, 0);
+  for (uint32_t e0 = 0; e0 < len; e0 += 64 * U) {
     uint32_t w[U];
     double pj[U][3], xj[U][3];
 #pragma unroll
     for (int q = 0; q < U; ++q) {
-      w[q] = e0 + q * 64 + lane < e1 ? wn[q] : 0u;  // mask 0: no term
-      const int64_t j = cb + (int64_t)(w[q] >> 8);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        pj[q][k] = xyz[3 * j + k];
-        xj[q][k] = (double)x[3 * j + k];
-      }
+      w[q] = wn[q];
+      const uint32_t j = cb + (w[q] >> 8);
+      load3(rp, 24u * j, xyz, pj[q]);
+      load3(rx, (uint32_t)(3 * sizeof(VT)) * j, x, xj[q]);
     }
 #pragma unroll
-    for (int q = 0; q < U; ++q) wn[q] = entries[min(e0 + 64 * U + q * 64 + lane, elast)];
+    for (int q = 0; q < U; ++q)
+      wn[q] = __builtin_amdgcn_raw_buffer_load_b32(rl, (int)(4u * (e0 + 64 * U + q * 64 + lane)), 0, 0);
     // all of this iteration's gathers and the next entries are in flight before any use
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1065,6 +1084,7 @@ inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const
   INCH_REQUIRE(n > 0 && xyz && cluster_atoms && cl_ptr && base && x && y && x != y,
                "inch_anm_matfree_cl: bad args");
   INCH_REQUIRE(ncl == (n + kClAtoms - 1) / kClAtoms, "inch_anm_matfree_cl: ncl must be ceil(n / 4)");
+  INCH_REQUIRE(24 * n <= (int64_t)UINT32_MAX, "inch_anm_matfree_cl: n too large for 32-bit offsets");
   INCH_REQUIRE(!qcoef || (Q && m_q > 0), "inch_anm_matfree_cl: qcoef needs Q");
   hipStream_t st = (hipStream_t)stream;
   const int code = config > 0 ? config : 2002;
