@@ -340,7 +340,8 @@ k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32
   if (c >= ncl) return;
   const int lane = threadIdx.x;
   double pi[kClAtoms][3], vi[kClAtoms][3];
-  double v[16];  // 4 atoms x 3 partial sums, padded to 16 for the halving reduction
+  __shared__ double red[3 * kClAtoms][65];  // cross-lane reduction (6.2 KB per wave)
+  double v[3 * kClAtoms];                   // 4 atoms x 3 partial sums
 #pragma unroll
   for (int t = 0; t < kClAtoms; ++t) {
     const int32_t at = cluster_atoms[kClAtoms * c + t];
@@ -352,12 +353,11 @@ k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32
     }
   }
 #pragma unroll
-  for (int q = 0; q < 16; ++q) v[q] = 0.0;
+  for (int q = 0; q < 3 * kClAtoms; ++q) v[q] = 0.0;
   // the epilogue's own loads depend only on the cluster's atoms: issue them before the list
-  const int idx = lane >> 2;
-  const bool writer = (lane & 3) == 0 && idx < 3 * kClAtoms;
-  const int32_t at_w = writer ? cluster_atoms[kClAtoms * c + idx / 3] : -1;
-  const int64_t dof = 3 * (int64_t)(at_w >= 0 ? at_w : 0) + idx % 3;
+  // (lane m < 12 writes value m: atom m / 3, component m % 3)
+  const int32_t at_w = lane < 3 * kClAtoms ? cluster_atoms[kClAtoms * c + lane / 3] : -1;
+  const int64_t dof = 3 * (int64_t)(at_w >= 0 ? at_w : 0) + lane % 3;
   double xv = 0.0, zv = 0.0, av = 0.0;
   if (at_w >= 0) {
     xv = (double)x[dof];
@@ -409,30 +409,22 @@ k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32
       }
     }
   }
-  // halving exchanges: at offset o the lane keeps one half of its values (upper half when
-  // lane & o) summed with its partner's copy of that half; after o = 32..4, lanes sharing
-  // bits 2..5 hold partial sums of value (lane >> 2); two full exchanges finish the sums.
-  // o = 32 and 16 are one v_permlane32/16_swap per dword; for o = 8 and 4 both halves are
-  // exchanged and the choice is made on the sums (a choice between two array elements would
-  // become a dynamically indexed register array).
+  // cross-lane sums through LDS in a fixed order: lanes 0..47 add 16 lanes' copies of value
+  // lane % 12 (quarter lane / 12), lanes 0..11 add the four quarters
 #pragma unroll
-  for (int q = 0; q < 8; ++q) v[q] = swap_sum<32>(v[q], v[q + 8]);
+  for (int m = 0; m < 3 * kClAtoms; ++m) red[m][lane] = v[m];
+  __syncthreads();
+  double sum = 0.0;
+  if (lane < 48) {
+    const int m = lane % 12, h = lane / 12;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) v[q] = swap_sum<16>(v[q], v[q + 4]);
-#pragma unroll
-  for (int o = 8, m = 2; o >= 4; o >>= 1, m >>= 1) {
-    const bool up = (lane & o) != 0;
-#pragma unroll
-    for (int q = 0; q < m; ++q) {
-      const double lo = v[q] + __shfl_xor(v[q], o, 64);
-      const double hi = v[q + m] + __shfl_xor(v[q + m], o, 64);
-      v[q] = up ? hi : lo;
-    }
+    for (int q = 0; q < 16; ++q) sum += red[m][16 * h + q];
   }
-  double sum = v[0];
-  sum += __shfl_xor(sum, 1, 64);
-  sum += __shfl_xor(sum, 2, 64);
-  if (at_w < 0) return;
+  __syncthreads();
+  red[0][lane] = sum;
+  __syncthreads();
+  if (at_w < 0) return;  // lanes >= 12 and padding slots
+  sum = red[0][lane] + red[0][lane + 12] + red[0][lane + 24] + red[0][lane + 36];
   double out = sum + plus_i * xv;
   if (c0 != 0.0) out -= c0 * xv;
   out *= s1;
