@@ -70,6 +70,9 @@ def spmv_only(args):
     cfg = {p: best_config(A, p) for p in ("f32", "f64", "mf64")}       # the launch shapes the solver uses
     l32, l64 = bsr_launch(A, cfg["f32"]), bsr_launch(A, cfg["f64"])
     out["kernels"] = {p: kernel_key(p, A) for p in cfg}
+    out["bytes_per_launch"] = {kernel_key(p, A): int(kernel_bytes(p, A, "cheb")) for p in cfg}
+    if A.clusters() is not None:
+        out["cluster_list_entries"] = int(A.clusters()[3].numel())
     cases = {
         "bsr_f32_cheb": lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x32, y32, s1=0.1, c0=2.0, z=z32,
                                              c2=1.0, acc=acc32, mu=0.01, lanes=l32[0], col16=l32[1]),
@@ -266,10 +269,8 @@ def kernel_bytes(kind, A, epi):
         return 9 * 8 * nnzb + (5 if epi == "cheb" else 2) * nd * 8
     vec = (5 if epi == "cheb" else 2) * nd * 8
     flags = best_config(A, kind) // 1000
-    if flags & 4:       # cluster layout: entries + cl_ptr + base + cluster_atoms
-        cl = A.clusters()
-        ncl = cl[2].numel()
-        return 4 * cl[3].numel() + 8 * (ncl + 1) + 4 * ncl + 32 * ncl + 24 * n_atoms + vec
+    if flags & 4:       # cluster layout: cluster_atoms + cl_ptr + base + list entries
+        return sum(t.numel() * t.element_size() for t in A.clusters()) + 24 * n_atoms + vec
     if flags & 1:       # 16-bit column offsets + row bases
         return 2 * nnzb + 8 * (n_atoms + 1) + 4 * n_atoms + 24 * n_atoms + vec
     return 4 * nnzb + 8 * (n_atoms + 1) + 24 * n_atoms + vec
@@ -365,12 +366,14 @@ def main():
     nnzb, n_atoms = A.nnzb, A.n_atoms
 
     def kernel_stats(timer, steps):
-        """Per-kernel totals from HIP event pairs around every operator launch."""
+        """Per-kernel totals from HIP event pairs on the launch stream: one pair around each
+        Chebyshev filter application (its operator launches run back to back, no other kernel
+        between them) and one around every other operator launch."""
         per = {}
-        for kind, epi, e0, e1 in timer:
+        for kind, epi, e0, e1, launches in timer:
             per.setdefault((kind, epi), [0.0, 0])
             per[(kind, epi)][0] += e0.elapsed_time(e1)
-            per[(kind, epi)][1] += 1
+            per[(kind, epi)][1] += launches
         return per
 
     def roofline(per, step_s, steps):
@@ -386,8 +389,9 @@ def main():
                 "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // steps,
                 "bytes_per_launch": int(by), "kernel_share_of_step": round(ms_total / steps / 1e3 / step_s, 3)}
 
-    # instrumented solve of the same workload: HIP events around every operator launch (the
-    # timed steps replay captured HIP graphs, where per-launch events cannot be recorded)
+    # instrumented solve of the same workload: HIP events around each filter application and
+    # each other operator launch (the timed steps replay captured HIP graphs, where events
+    # cannot be recorded between launches)
     timer = []
     torch.cuda.synchronize()
     ti = time.time()

@@ -381,7 +381,8 @@ __global__ void __launch_bounds__(256) k_assemble(int64_t n, const double *__res
 // ---- cluster layout of the matrix-free operator --------------------------------------
 // Atoms are grouped into clusters of kClusterAtoms that are compact in space: cells of
 // cell_xy x cell_xy x cell_z, numbered column-major (x, y column, then z inside the column),
-// atoms in a cell by index; cluster c holds atoms [4c, 4c + 4) of that order. Each cluster
+// atoms in a cell by index, cut into groups of 4; the groups are numbered by their smallest
+// neighbour index, so consecutive clusters gather from one band of the vectors. Each cluster
 // lists the union of its atoms' neighbour rows once, as (j, mask) with mask bit t set when j
 // is a neighbour of the cluster's atom t (j itself excluded), so the product gathers a
 // neighbour's coordinates and vector entries once per cluster instead of once per atom.
@@ -550,6 +551,19 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const int64_t *__
   int r = 0;
   for (int e = e0; e < e1; ++e)
     if (val[e]) entries[out + r++] = (((uint32_t)(key[e] >> 3) - b0) << 8) | (uint32_t)val[e];
+}
+
+// Clusters renumbered: slot c' takes cluster perm[c'] (atoms, list length, base).
+__global__ void k_cl_permute(int64_t ncl, const int32_t *__restrict__ perm, const int32_t *__restrict__ atoms_in,
+                             const int64_t *__restrict__ counts_in, const int32_t *__restrict__ base_in,
+                             int32_t *__restrict__ atoms_out, int64_t *__restrict__ counts_out,
+                             int32_t *__restrict__ base_out) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  const int64_t src = perm[c];
+  for (int t = 0; t < kClusterAtoms; ++t) atoms_out[kClusterAtoms * c + t] = atoms_in[kClusterAtoms * src + t];
+  counts_out[c] = counts_in[src];
+  base_out[c] = base_in[src];
 }
 
 struct DevBuf {
@@ -854,8 +868,33 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   const int64_t slots = ncl * kClusterAtoms;
   k_cl_atoms<<<(unsigned)((slots + bs - 1) / bs), bs, 0, st>>>(n, slots, (int32_t *)atoms.p, cluster_atoms);
   HIP_TRY(hipGetLastError());
-  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, cluster_atoms, nullptr, base,
-                                                   (int64_t *)counts.p, nullptr, (int32_t *)ovf.p);
+  DevBuf atoms0, counts0, base0, perm;
+  HIP_TRY(hipMalloc(&atoms0.p, slots * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&counts0.p, ncl * sizeof(int64_t)));
+  HIP_TRY(hipMalloc(&base0.p, ncl * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&perm.p, ncl * sizeof(int32_t)));
+  HIP_TRY(hipMemcpyAsync(atoms0.p, cluster_atoms, slots * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
+                                                   (int64_t *)counts0.p, nullptr, (int32_t *)ovf.p);
+  HIP_TRY(hipGetLastError());
+  {
+    inch_status s0 = check_overflow("count");
+    if (s0 != INCH_OK) return s0;
+  }
+  // clusters in the order of their smallest neighbour (ties by spatial order): consecutive
+  // clusters, and so each XCD's contiguous share of them, gather from one band of the
+  // vectors, which are in the matrix's (RCM) order
+  std::vector<int32_t> hb(ncl), order(ncl);
+  HIP_TRY(hipMemcpyAsync(hb.data(), base0.p, ncl * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<uint64_t> key(ncl);
+  for (int64_t c = 0; c < ncl; ++c) key[c] = ((uint64_t)(uint32_t)hb[c] << 32) | (uint64_t)c;
+  std::sort(key.begin(), key.end());
+  for (int64_t c = 0; c < ncl; ++c) order[c] = (int32_t)(key[c] & 0xffffffffu);
+  HIP_TRY(hipMemcpyAsync(perm.p, order.data(), ncl * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  k_cl_permute<<<(unsigned)((ncl + bs - 1) / bs), bs, 0, st>>>(ncl, (int32_t *)perm.p, (int32_t *)atoms0.p,
+                                                              (int64_t *)counts0.p, (int32_t *)base0.p, cluster_atoms,
+                                                              (int64_t *)counts.p, base);
   HIP_TRY(hipGetLastError());
   k_scan_i64<<<1, 1024, 0, st>>>(ncl, (int64_t *)counts.p, cl_ptr);
   HIP_TRY(hipGetLastError());

@@ -203,8 +203,9 @@ inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *st
  * inch_anm_matfree_spmv reads (same reference interfaces). The atoms are grouped into
  * ncl = ceil(n / 4) clusters that are compact in space: cells of cell_xy x cell_xy x cell_z
  * (z cells coarsened until there are at most max(2n, 65536) cells), numbered by x, y column
- * then z, atoms in a cell by index; cluster c holds atoms [4c, 4c + 4) of that order,
- * cluster_atoms int32[ncl * 4] (-1 pads the last cluster). Each cluster lists the union of its
+ * then z, atoms in a cell by index, cut into groups of 4; the groups are then numbered by
+ * their smallest neighbour index (ties by spatial order), so consecutive clusters gather
+ * from one band of the vectors; cluster_atoms int32[ncl * 4] (-1 pads the last group). Each cluster lists the union of its
  * atoms' rows once: entries uint32 = ((j - base[c]) << 8) | mask, ascending j, mask bit t set
  * when j is a neighbour of atom t of the cluster (j itself excluded); cl_ptr int64[ncl + 1],
  * base int32[ncl]. Two passes, as inch_neighbors_celllist: with entries == NULL it writes
