@@ -36,10 +36,16 @@ class ModeResult:
         return {k: v for k, v in self.__dict__.items() if not isinstance(v, torch.Tensor)}
 
 
+STOP_SAFETY = 0.3   # a cycle ends when the predicted worst relative residual is 0.3 x the target
+
+
 def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
     """TRLM callback: float64 RR polish of the current Ritz block; stop when all k pairs
     meet ``tol_rel`` or (``stall`` given) when the worst residual improves by less than
-    that factor between restarts."""
+    that factor between restarts. Each polish also calibrates the ratio between the worst
+    true relative residual and the Lanczos estimate max |beta_k| of the same restart, and
+    sets the solver's in-cycle threshold so that a cycle ends once the calibrated estimate
+    predicts STOP_SAFETY x tol_rel."""
     state = {"calls": 0, "last": None, "hist": [], "armed": False}
 
     def check(trlm, S):
@@ -60,15 +66,22 @@ def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
         h = state["hist"]
         if stall is not None and len(h) >= 2 and h[-1] > stall * h[-2]:
             return True
+        if trlm.last_est and worst > 0.0:
+            trlm.stop_est = STOP_SAFETY * tol_rel * trlm.last_est / worst
         return False
 
     return check, state
 
 
 def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, bound=None, wanted_right=None,
-                 max_degree=5120, min_degree=5, ratio=0.7, hotelling=None, factor=2, maxiter=200, seed=0,
+                 max_degree=5120, min_degree=5, ratio=0.7, hotelling=None, factor=3, maxiter=200, seed=0,
                  phase1_rel=3e-7, v0=None, verbose=False, phase2="mf64", timer=None, kpm_count=None):
     """k lowest non-trivial eigenpairs of A = H + PlusI I (RCM order) on the device.
+
+    ``factor`` sets the Lanczos basis to factor * k vectors (the reference's
+    User_WorkspaceSizeFactor); 3 is the default here because with restart cycles that end
+    once the residual estimates predict the target, fewer, longer cycles took 6-8% less
+    time than factor 2 on 300k-atom globules (64 modes, two seeds).
 
     Returns ModeResult with eigval (ascending, includes PlusI), eigvec rows [k, n] (float64
     device), rel_res, per-phase seconds and SpMV counts."""
@@ -111,7 +124,8 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
         # or stops improving (the float32 floor)
         check1, state1 = _check_factory(A, Q, lam_max, phase1_rel, k, stall=0.7)
         r1 = TRLM(op32, k, factor=factor).run(maxiter=maxiter, tol=0.0, v0=start, seed=seed, filtered=filtered,
-                                                check=check1, verbose=verbose)
+                                                check=check1, verbose=verbose,
+                                                stop_est=STOP_SAFETY * phase1_rel * (1.0 if filtered else lam_max))
         phases["phase1_f32"] = time.time() - t
         n_spmv["phase1_f32"] = op32.n_spmv
         total += op32.n_spmv
@@ -132,8 +146,11 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
     t = time.time()
     p2 = phase2 if precision == "mixed" else precision
     op64 = Operator(A, p2, pol=pol, Q=Q, shift=hotelling, timer=timer if precision != "mixed" else None)
+    # until a polish calibrates it, the in-cycle stop assumes the Lanczos estimate of the
+    # filtered operator equals the relative residual (measured: it is about 3x larger)
     r2 = TRLM(op64, k, factor=factor).run(maxiter=maxiter, tol=0.0, v0=start, seed=seed, filtered=filtered,
-                                           check=check, verbose=verbose)
+                                           check=check, verbose=verbose,
+                                           stop_est=STOP_SAFETY * tol_rel * (1.0 if filtered else lam_max))
     key = ("phase2_" + p2) if precision == "mixed" else ("solve_" + p2)
     phases[key] = time.time() - t
     n_spmv[key] = op64.n_spmv

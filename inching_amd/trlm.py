@@ -43,13 +43,23 @@ class TRLMResult:
         self.history = history
 
 
+CYCLE_SEGMENT = 16   # Lanczos steps per captured segment of a restart cycle
+
+
 class TRLM:
     """Device TRLM on an ``Operator`` (see operator.py).
 
-    ``use_graph``: the restart body (restart vector, first extension step and the Lanczos
-    sweep k+1..ncv-1) launches the same kernels on the same buffers every restart, so it
-    is captured once into a HIP graph and replayed; only beta_k is written from the host
-    before each replay."""
+    ``use_graph``: a restart cycle (restart vector, first extension step and the Lanczos
+    sweep k+1..ncv-1) launches the same kernels on the same buffers every restart, so it is
+    captured once into HIP graphs, one per segment of CYCLE_SEGMENT steps, and replayed;
+    only beta_k is written from the host before a cycle.
+
+    A cycle may end after any segment (the thick restart of the reference, from the m basis
+    vectors built so far): with ``stop_est`` set, the projected matrix after each segment
+    gives the Lanczos residual estimates |beta_m s_m,i| of the wanted Ritz pairs. The
+    estimate e_s of segment s is computed on the host from an asynchronous copy of alpha /
+    beta while segment s + 1 runs; the cycle ends after segment s + 1 once e_s <= stop_est
+    or the geometric extrapolation e_s^2 / e_(s-1) to segment s + 1 is <= 0.3 stop_est."""
 
     def __init__(self, op, k, factor=2, ncv=None, use_graph=True):
         n = op.n
@@ -58,7 +68,7 @@ class TRLM:
         self.op = op
         self.k = int(k)
         self.n = n
-        self.ncv = int(ncv) if ncv else min(max(factor * self.k, self.k + 32), n - 1)
+        self.ncv = int(ncv) if ncv else min(max(int(round(factor * self.k)), self.k + 32), n - 1)
         dev = op.A.device
         self.V = torch.empty((self.ncv, n), dtype=torch.float64, device=dev)
         self.Vk = torch.empty((self.k, n), dtype=torch.float64, device=dev)
@@ -71,17 +81,22 @@ class TRLM:
         self.ws = ops.Workspace(self.ncv + 1, dev)
         self.cgs_ws = ops.CGS2Workspace(min(self.ncv, ops.CGS2_MAX_J), dev)
         self.use_graph = bool(use_graph) and op.timer is None
-        self.graph = None
-        self._graph_counts = (0, 0)
+        self.graphs = {}
+        self.stop_est = None    # residual-estimate threshold that ends a cycle early
+        self.steps = 0          # Lanczos steps taken so far
+        self.last_est = None    # max |beta_k| of the last restart
+        self._ah = torch.empty(self.ncv, dtype=torch.float64, pin_memory=True)
+        self._bh = torch.empty(self.ncv, dtype=torch.float64, pin_memory=True)
 
-    # -- Lanczos steps i_start .. i_end-1 (the last step leaves its residual in u)
+    # -- Lanczos steps i_start .. i_end-1: every step leaves its residual in u and, while the
+    # basis has room (i + 1 < ncv), the next basis vector u / beta_i in V[i + 1]
     def lanczos(self, i_start, i_end):
         V, u, h, ws = self.V, self.u, self.h, self.ws
         for i in range(i_start, i_end):
             self.op.apply(V[i], u)
             ops.multidot(V, i + 1, u, h, ws)
             self.alpha[i:i + 1].copy_(h[i:i + 1])
-            last = i >= i_end - 1
+            last = i >= self.ncv - 1
             if i + 1 <= ops.CGS2_MAX_J:
                 # fused: update + second-pass dots in one read of V, then update + normalise
                 ops.cgs2_reorth(V, i + 1, h, u, self.beta[i:i + 1], vnext=None if last else V[i + 1],
@@ -94,49 +109,91 @@ class TRLM:
                 break
             ops.scale_div(u, self.beta[i:i + 1], V[i + 1])
 
-    def _body(self):
-        """Restart vector (orthogonalised against the k Ritz vectors), the arrowhead
-        extension step and the Lanczos sweep k+1 .. ncv-1."""
-        k, ncv = self.k, self.ncv
-        V, u, ws = self.V, self.u, self.ws
-        ops.multidot(V, k, u, self.h, ws)
-        ops.gemv_update(V, k, self.h, u, 1.0, nrm2_out=self.scal, ws=ws)
-        ops.scale_div(u, self.scal, V[k])
-        self.op.apply(V[k], u)
-        ops.multidot(V[k:k + 1], 1, u, self.alpha[k:k + 1], ws)
-        ops.axpy_dev(self.alpha[k:k + 1], -1.0, V[k], u)
-        ops.gemv_update(V, k, self.betak_d, u, 1.0, nrm2_out=self.beta[k:k + 1], ws=ws)
-        ops.scale_div(u, self.beta[k:k + 1], V[k + 1])
-        self.lanczos(k + 1, ncv)
+    def _segments(self):
+        """Step ranges of a restart cycle: k+1 .. ncv-1 in pieces of CYCLE_SEGMENT steps."""
+        b = list(range(self.k + 1, self.ncv, CYCLE_SEGMENT)) + [self.ncv]
+        return list(zip(b[:-1], b[1:]))
 
-    def _run_body(self):
+    def _segment(self, s, lo, hi):
+        """Segment s of a restart cycle; the first one starts with the restart vector
+        (orthogonalised against the k Ritz vectors) and the arrowhead extension step."""
+        if s == 0:
+            k = self.k
+            V, u, ws = self.V, self.u, self.ws
+            ops.multidot(V, k, u, self.h, ws)
+            ops.gemv_update(V, k, self.h, u, 1.0, nrm2_out=self.scal, ws=ws)
+            ops.scale_div(u, self.scal, V[k])
+            self.op.apply(V[k], u)
+            ops.multidot(V[k:k + 1], 1, u, self.alpha[k:k + 1], ws)
+            ops.axpy_dev(self.alpha[k:k + 1], -1.0, V[k], u)
+            ops.gemv_update(V, k, self.betak_d, u, 1.0, nrm2_out=self.beta[k:k + 1], ws=ws)
+            ops.scale_div(u, self.beta[k:k + 1], V[k + 1])
+        self.lanczos(lo, hi)
+
+    def _run_segment(self, s, lo, hi):
         if not self.use_graph:
-            self._body()
+            self._segment(s, lo, hi)
             return
-        if self.graph is None:
+        if s not in self.graphs:
             n0, a0 = self.op.n_spmv, self.op.n_apply
             g = torch.cuda.CUDAGraph()
             with torch.cuda.graph(g):
-                self._body()
-            self.graph = g
-            self._graph_counts = (self.op.n_spmv - n0, self.op.n_apply - a0)
+                self._segment(s, lo, hi)
+            self.graphs[s] = (g, self.op.n_spmv - n0, self.op.n_apply - a0)
         else:
-            self.op.n_spmv += self._graph_counts[0]
-            self.op.n_apply += self._graph_counts[1]
-        self.graph.replay()
+            self.op.n_spmv += self.graphs[s][1]
+            self.op.n_apply += self.graphs[s][2]
+        self.graphs[s][0].replay()
+
+    def _run_cycle(self, S, beta_k, filtered):
+        """One restart cycle; returns the number m of basis vectors built (alpha[:m],
+        beta[:m] hold the projected matrix, u the residual)."""
+        k = self.k
+        segs = self._segments()
+        pending = None   # (m, event) of the last segment whose alpha / beta copy is in flight
+        prev = None      # estimate of the segment before it
+        for s, (lo, hi) in enumerate(segs):
+            self._run_segment(s, lo, hi)
+            if hi >= self.ncv:
+                return hi
+            if self.stop_est is None:
+                continue
+            if pending is not None:
+                m, ev = pending
+                ev.synchronize()
+                a = self._ah[:m].numpy().copy()
+                b = self._bh[:m].numpy().copy()
+                a[:k] = S
+                b[:k] = 0.0
+                _, W = select_ritz(a, b, beta_k, k, filtered)
+                est = float(np.max(np.abs(b[-1] * W[-1, :])))
+                if est <= self.stop_est or (prev is not None and est < prev
+                                            and est * est / prev <= 0.3 * self.stop_est):
+                    return hi
+                prev = est
+            self._ah[:hi].copy_(self.alpha[:hi], non_blocking=True)
+            self._bh[:hi].copy_(self.beta[:hi], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            pending = (hi, ev)
+        return self.ncv
 
     def _restart_rr(self, alpha_h, beta_h, beta_k, filtered):
         S, W = select_ritz(alpha_h, beta_h, beta_k, self.k, filtered)
         Wd = torch.as_tensor(np.ascontiguousarray(W), device=self.V.device)
-        ops.rr_combine(self.V, self.ncv, Wd, self.Vk)
+        ops.rr_combine(self.V, len(alpha_h), Wd, self.Vk)
         self.V[:self.k].copy_(self.Vk)
         beta_k = beta_h[-1] * W[-1, :]
+        self.last_est = float(np.max(np.abs(beta_k)))
         return S, beta_k, float(np.linalg.norm(beta_k))
 
-    def run(self, maxiter=10000, tol=0.0, v0=None, seed=0, filtered=False, check=None, verbose=False):
+    def run(self, maxiter=10000, tol=0.0, v0=None, seed=0, filtered=False, check=None, verbose=False,
+            stop_est=None):
         """Returns TRLMResult. ``check(self, S)`` (optional) is called after every restart and
-        may return True to stop (residual-verified acceptance); ``tol`` is the reference's
-        absolute criterion on ||beta_k||."""
+        may return True to stop (residual-verified acceptance; it may also update
+        ``self.stop_est``); ``tol`` is the reference's absolute criterion on ||beta_k||;
+        ``stop_est`` (optional) ends restart cycles early, see the class docstring."""
+        self.stop_est = stop_est
         t0 = time.time()
         k, ncv = self.k, self.ncv
         if tol == 0:
@@ -149,6 +206,7 @@ class TRLM:
             self.u.copy_(torch.as_tensor(v0, dtype=torch.float64, device=self.V.device).reshape(-1))
         self.V[0].copy_(self.u / torch.linalg.norm(self.u))
         self.lanczos(0, ncv)
+        self.steps = ncv
         alpha_h = self.alpha.cpu().numpy().copy()
         beta_h = self.beta.cpu().numpy().copy()
         S, beta_k, res = self._restart_rr(alpha_h, beta_h, None, filtered)
@@ -162,9 +220,10 @@ class TRLM:
             if res <= tol or stop:
                 break
             self.betak_d.copy_(torch.as_tensor(beta_k, device=self.V.device))
-            self._run_body()
-            alpha_h = self.alpha.cpu().numpy().copy()
-            beta_h = self.beta.cpu().numpy().copy()
+            m = self._run_cycle(S, beta_k, filtered)
+            self.steps += m - k
+            alpha_h = self.alpha[:m].cpu().numpy().copy()
+            beta_h = self.beta[:m].cpu().numpy().copy()
             alpha_h[:k] = S
             beta_h[:k] = 0.0
             S, beta_k, res = self._restart_rr(alpha_h, beta_h, beta_k, filtered)
