@@ -299,6 +299,22 @@ def kernel_key(kind, A):
     return "k_bsr_spmv<%s,%s,%d,int,false>" % (t, t, cfg)
 
 
+def measured_valu_issue(key):
+    """Share of the chip's VALU issue slots the kernel ``key`` fills, from the committed
+    rocprofv3 SQ-counter summary (SQ_INSTS_VALU x 4 cycles / (SIMDs x duration)), or None."""
+    path = os.path.join(ROOT, "profiles", "matfree_kernels_globule300k_pmc_sq.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("k_"):
+                    name = line[:line.index(">") + 1].replace(", ", ",")
+                    if name == key:
+                        return float(line[line.index(">") + 1:].split()[5])
+    except (OSError, ValueError, IndexError):
+        pass
+    return None
+
+
 def measured_traffic(key):
     """HBM bytes per launch of ``key`` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (calibrated on a copy kernel), or None when that kernel was not counted."""
@@ -385,6 +401,7 @@ def main():
         ach = by / (avg_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_key(kind, A)),
+                "valu_issue_frac": measured_valu_issue(kernel_key(kind, A)),
                 "kernel": kernel_key(kind, A) + " (" + KERNEL_DESC[kind] + ")",
                 "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // steps,
                 "bytes_per_launch": int(by), "kernel_share_of_step": round(ms_total / steps / 1e3 / step_s, 3)}
