@@ -64,6 +64,7 @@ SIGNATURES = {
     "inch_seg_update": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, _dbl, _vp, _vp, _vp,
                                   _vp]),
     "inch_seg_scale": (_STATUS, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_seg_cgs_b": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "inch_seg_combine": (_STATUS, [_i32, _vp, _vp, _vp, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
     "inch_seg_resnorm": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i64, _vp, _vp, _vp,
                                    _vp]),

@@ -60,7 +60,17 @@ SP = ops.stream_ptr
 class SegOps:
     def __init__(self, seg, maxJ, device):
         self.seg = seg
+        self.maxJ = int(maxJ)
         self.wk = seg.work(maxJ, device)
+
+    def cgs_b(self, V, J, h, hld, u, h2, nrm):
+        """u -= V[:J] h per segment; h2[s] = (V[:J] u, |u|^2) over segment s; nrm[s] = |u| (one
+        read of V)."""
+        s = self.seg
+        if J + 1 > self.maxJ:
+            raise ValueError("cgs_b: workspace holds J + 1 <= %d partials per chunk" % self.maxJ)
+        call("inch_seg_cgs_b", s.nchunk, s.S, P(s.cbeg), P(s.cend), P(s.cseg), P(s.sptr), int(J), P(V), V.stride(0),
+             P(h), int(hld), P(u), P(h2), P(nrm), P(self.wk), SP())
 
     def multidot(self, V, J, u, h):
         s = self.seg
@@ -206,6 +216,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     Vk = torch.empty((k, n), **f64)
     u = torch.empty(n, **f64)
     h = torch.empty(S * (ncv + 1), **f64)
+    h2 = torch.empty(S * (ncv + 1), **f64)
     hq = torch.empty(S * 6, **f64)
     alpha = torch.zeros((ncv, S), **f64)
     beta = torch.zeros((ncv, S), **f64)
@@ -224,9 +235,10 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
             J = i + 1
             so.multidot(V, J, u, h)
             alpha[i].copy_(h[:S * J].view(S, J)[:, i])
-            so.update(V, J, h, J, 1.0, u, nrm=beta[i])
-            so.multidot(V, J, u, h)
-            so.update(V, J, h, J, 1.0, u)
+            # CGS2: update + second projection in one read of V (beta from the first update),
+            # then the second update
+            so.cgs_b(V, J, h, J, u, h2, beta[i])
+            so.update(V, J, h2, J + 1, 1.0, u)
             if i < i1 - 1:
                 so.scale(u, beta[i], V[i + 1])
 

@@ -85,6 +85,72 @@ __global__ void __launch_bounds__(256) k_seg_update(const int64_t *__restrict__ 
   }
 }
 
+// The update and the second projection of CGS2 in one read of V, per segment:
+// u' = u - V[:J] h (h of the chunk's segment, row stride hld) and, per chunk c,
+// part[c*(J+1) + j] = V[j, chunk] . u'[chunk] (j < J), part[c*(J+1) + J] = |u'[chunk]|^2.
+// Tiles of C dofs of V[:J] are staged in LDS (J <= 4096 / C): 256 / C lanes per column form
+// u' with a fixed butterfly, then thread j accumulates row j's dot over the tile.
+template <int C>
+__global__ void __launch_bounds__(256) k_seg_cgs_b(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
+                                                    const int32_t *__restrict__ cseg, int J,
+                                                    const double *__restrict__ V, int64_t ld,
+                                                    const double *__restrict__ h, int hld, double *__restrict__ u,
+                                                    double *__restrict__ part) {
+  constexpr int LPC = 256 / C;         // lanes per column
+  constexpr int RS = 256 / C;          // rows staged per pass of the workgroup
+  constexpr int NR = (4096 / C) / RS;  // staging loads per thread
+  __shared__ double vs[4096 / C][C + 1];
+  __shared__ double hs[4096 / C];
+  __shared__ double up[C];
+  const int tid = threadIdx.x;
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  const int64_t b = cbeg[c], e = cend[c];
+  for (int t = tid; t < J; t += 256) hs[t] = h[(int64_t)s * hld + t];
+  double acc = 0.0, ss = 0.0;
+  const int c_ld = tid % C, r_ld = tid / C;
+  for (int64_t c0 = b; c0 < e; c0 += C) {
+    __syncthreads();
+    const int64_t il = c0 + c_ld;
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+      const int r = r_ld + q * RS;
+      if (r < J) vs[r][c_ld] = il < e ? V[(int64_t)r * ld + il] : 0.0;
+    }
+    __syncthreads();
+    {
+      const int col = tid / LPC, pl = tid % LPC;
+      double sum = 0.0;
+      for (int r = pl; r < J; r += LPC) sum += vs[r][col] * hs[r];
+#pragma unroll
+      for (int off = LPC / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LPC);
+      if (pl == 0) {
+        const int64_t i = c0 + col;
+        const double un = i < e ? u[i] - sum : 0.0;
+        up[col] = un;
+        if (i < e) u[i] = un;
+      }
+    }
+    __syncthreads();
+    if (tid < J) {
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc) acc += vs[tid][cc] * up[cc];
+    }
+    if (tid == 0) {
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc) ss += up[cc] * up[cc];
+    }
+  }
+  if (tid < J) part[(int64_t)c * (J + 1) + tid] = acc;
+  if (tid == 0) part[(int64_t)c * (J + 1) + J] = ss;
+}
+
+// dst[s] = sqrt(src[s * stride + off])
+__global__ void k_seg_sqrt_col(int S, const double *__restrict__ src, int stride, int off, double *__restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < S) dst[t] = sqrt(src[(int64_t)t * stride + off]);
+}
+
 // dst[i] = src[i] / div[s] (dst2 too when given)
 __global__ void __launch_bounds__(256) k_seg_scale(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
                                                     const int32_t *__restrict__ cseg, const double *__restrict__ src,
@@ -180,6 +246,28 @@ inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, cons
     k_seg_reduce<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, 1, sptr, work, nrm_out, 1);
     HIP_TRY(hipGetLastError());
   }
+  return INCH_OK;
+}
+
+inch_status inch_seg_cgs_b(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                           const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
+                           double *u, double *h2, double *nrm_out, double *work, void *stream) {
+  INCH_REQUIRE(nchunk > 0 && S > 0 && J > 0 && J <= 256 && hld >= J && cbeg && cend && cseg && sptr && V && h && u &&
+                   h2 && nrm_out && work,
+               "inch_seg_cgs_b: bad args (0 < J <= 256)");
+  hipStream_t st = (hipStream_t)stream;
+  if (J <= 64)
+    k_seg_cgs_b<64><<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, J, V, ld, h, hld, u, work);
+  else if (J <= 128)
+    k_seg_cgs_b<32><<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, J, V, ld, h, hld, u, work);
+  else
+    k_seg_cgs_b<16><<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, J, V, ld, h, hld, u, work);
+  HIP_TRY(hipGetLastError());
+  const int64_t tot = (int64_t)S * (J + 1);
+  k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, J + 1, sptr, work, h2, 0);
+  HIP_TRY(hipGetLastError());
+  k_seg_sqrt_col<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, h2, J + 1, J, nrm_out);
+  HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
 

@@ -312,6 +312,13 @@ inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, cons
                             const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
                             double scale, double *u, double *nrm_out, double *work, void *stream);
 /* dst[i] = src[i] / div[s] (and dst2). */
+/* CGS2's update and second projection in one read of V, per segment: u -= V[:J] h (h of
+ * each segment, row stride hld), then h2[s, j] = V[j] . u over segment s (j < J),
+ * h2[s, J] = |u|^2 over s and nrm_out[s] = sqrt(h2[s, J]); h2 is [S, J + 1]. work holds
+ * nchunk * (J + 1) partials. J <= 256. */
+inch_status inch_seg_cgs_b(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
+                           const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
+                           double *u, double *h2, double *nrm_out, double *work, void *stream);
 inch_status inch_seg_scale(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
                            const double *src, const double *div, double *dst, double *dst2, void *stream);
 /* out[q*ld_out+i] = sum_r W[s][r][q] V[r*ld+i] (W [S, m, k], k <= 64, m*k <= 8192). */

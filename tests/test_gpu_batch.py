@@ -55,6 +55,36 @@ def test_batched_segmented_kernels(cuda):
         assert np.allclose(out.cpu().numpy()[:, sl], W[s].T @ V[:, sl], rtol=1e-11, atol=1e-10)
 
 
+@pytest.mark.parametrize("J", [11, 70, 150])
+def test_segmented_fused_cgs_pass(cuda, J):
+    """u -= V h per segment, second projection and norm in one pass (three tile widths)."""
+    import torch
+    from inching_amd.batch import SegOps, Segments
+    rng = np.random.default_rng(J)
+    seg_ptr = np.array([0, 5000, 5003, 16000, 30001])
+    n = seg_ptr[-1]
+    seg = Segments(seg_ptr, torch.device("cuda"), chunk=1024)
+    so = SegOps(seg, J + 1, torch.device("cuda"))
+    V = rng.standard_normal((J, n))
+    u = rng.standard_normal(n)
+    hh = rng.standard_normal((4, J))
+    Vd, ud = torch.as_tensor(V, device="cuda"), torch.as_tensor(u, device="cuda")
+    h2 = torch.empty(4 * (J + 1), dtype=torch.float64, device="cuda")
+    nrm = torch.empty(4, dtype=torch.float64, device="cuda")
+    so.cgs_b(Vd, J, torch.as_tensor(hh.reshape(-1), device="cuda"), J, ud, h2, nrm)
+    uref = u.copy()
+    for s in range(4):
+        sl = slice(seg_ptr[s], seg_ptr[s + 1])
+        uref[sl] -= V[:, sl].T @ hh[s]
+    assert np.allclose(ud.cpu().numpy(), uref, rtol=1e-11, atol=1e-10)
+    got = h2.cpu().numpy().reshape(4, J + 1)
+    for s in range(4):
+        sl = slice(seg_ptr[s], seg_ptr[s + 1])
+        assert np.allclose(got[s, :J], V[:, sl] @ uref[sl], rtol=1e-10, atol=1e-8)
+        assert np.isclose(got[s, J], uref[sl] @ uref[sl], rtol=1e-12)
+        assert np.isclose(nrm.cpu().numpy()[s], np.linalg.norm(uref[sl]), rtol=1e-12)
+
+
 @pytest.mark.parametrize("filtered,per_component", [(False, 12), (True, 12), (True, 4)])
 def test_assembly_merge_matches_union_of_component_spectra(cuda, filtered, per_component):
     """Disconnected assembly: the lowest modes are the merged per-subunit spectra (the

@@ -496,3 +496,45 @@ def test_cluster_apply_multi_rhs(mat, nrhs):
     Yr = torch.empty_like(Y)
     ops.matfree_apply(A.xyz, A.rowptr, A.col, A.plus_i, _dev(V), Yr)
     assert torch.allclose(Y, Yr, rtol=0, atol=1e-12 * float(np.abs(ref).max()))
+
+
+def test_cluster_layout_with_padding():
+    """n not a multiple of 4: the last cluster has padding slots (-1), which must neither
+    read nor write; layout and products still match the CSR and the oracle."""
+    from inching_amd import ops
+    from inching_amd.hessian import build_hessian
+    X = small_globule(1003, seed=23)
+    _print("case: cluster layout with padding, n=1003")
+    A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=())
+    cl = A.clusters()
+    assert cl is not None
+    catoms, lists = _decode_clusters(cl)
+    flat = catoms.reshape(-1)
+    assert flat.size == 1004 and np.sum(flat == -1) == 1
+    assert np.array_equal(np.sort(flat[flat >= 0]), np.arange(1003))
+    pairs = []
+    for c, (j, m) in enumerate(lists):
+        for t in range(4):
+            sel = (m >> t) & 1 == 1
+            if sel.any():
+                assert catoms[c, t] >= 0
+                pairs.append(catoms[c, t] * np.int64(1003) + j[sel])
+    rp, col = A.rowptr.cpu().numpy(), A.col.cpu().numpy().astype(np.int64)
+    rows = np.repeat(np.arange(1003), np.diff(rp))
+    off = rows != col
+    assert np.array_equal(np.sort(np.concatenate(pairs)), np.sort(rows[off] * np.int64(1003) + col[off]))
+    Xr = A.xyz.cpu().numpy()
+    rpo, colo, vals = OH.block_hessian(Xr, 8.0, 1.0)
+    Bo = OH.block_to_scipy(rpo, colo, vals, 1003)
+    rng = np.random.default_rng(3)
+    x, z = rng.standard_normal(3009), rng.standard_normal(3009)
+    out = 0.7 * (Bo @ x - 1.5 * x) - z
+    for cfg in (4001, 6001, 6002, 6004):
+        y = torch.full((3009,), np.nan, dtype=torch.float64, device="cuda")
+        A.matfree_spmv(_dev(x), y, config=cfg, s1=0.7, c0=1.5, z=_dev(z), c2=1.0)
+        assert np.abs(y.cpu().numpy() - out).max() <= 1e-12 * np.abs(out).max(), cfg
+    V = rng.standard_normal((3009, 5))
+    Y = torch.full((3009, 5), np.nan, dtype=torch.float64, device="cuda")
+    ops.matfree_cl_apply(A.xyz, cl, A.plus_i, _dev(V), Y)
+    ref = Bo @ V
+    assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-12 * np.abs(ref).max()
