@@ -161,11 +161,28 @@ __global__ void k_rcm_place(int32_t e, int32_t L1, const int32_t *__restrict__ o
   if (q < L1) pos[ord[e + q]] = e + q;
 }
 
-// First k >= z with pos[seeds[k]] < 0 (atomicMin into *best, ULLONG_MAX when none).
-__global__ void k_rcm_next_seed(int64_t n, int64_t z, const int64_t *__restrict__ seeds,
-                                const int32_t *__restrict__ pos, unsigned long long *__restrict__ best) {
+// First k >= z with pos[seeds[k]] < 0 (atomicMin into *best, ULLONG_MAX when none). The
+// block's minimum is formed first (wave shuffles, then the block's waves), so a block does
+// at most one atomic and blocks with no unvisited seed none: with many components almost
+// every entry past z is still unvisited, and one atomic per entry serialised on *best.
+__global__ void __launch_bounds__(256) k_rcm_next_seed(int64_t n, int64_t z, const int64_t *__restrict__ seeds,
+                                                       const int32_t *__restrict__ pos,
+                                                       unsigned long long *__restrict__ best) {
+  __shared__ unsigned long long wmin[4];
   const int64_t k = z + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < n && pos[seeds[k]] < 0) atomicMin(best, (unsigned long long)k);
+  unsigned long long cand = (k < n && pos[seeds[k]] < 0) ? (unsigned long long)k : ULLONG_MAX;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(cand, off, 64);
+    cand = o < cand ? o : cand;
+  }
+  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = cand;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = wmin[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmin[w] < m ? wmin[w] : m;
+    if (m != ULLONG_MAX) atomicMin(best, m);
+  }
 }
 
 __global__ void k_rcm_start(int32_t N, const int64_t *__restrict__ seeds, const unsigned long long *__restrict__ best,
