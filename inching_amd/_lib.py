@@ -101,9 +101,14 @@ def lib():
     return _lib
 
 
+_fns = {}
+
+
 def call(name, *args):
     """Call an ``inch_*`` entry point and raise InchingError on a non-zero status."""
-    fn = getattr(lib(), name)
+    fn = _fns.get(name)
+    if fn is None:
+        fn = _fns[name] = getattr(lib(), name)
     st = fn(*args)
     if SIGNATURES[name][0] is _STATUS and st != 0:
         msg = lib().inch_last_error()

@@ -19,15 +19,18 @@ def require_gpu():
 
 
 def stream_ptr(stream=None):
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return C.c_void_p(s.cuda_stream)
+    """hipStream_t of ``stream``, or of torch's current stream (the raw accessor avoids
+    constructing a Stream object, which costs several microseconds per launch)."""
+    if stream is not None:
+        return stream.cuda_stream
+    return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
 
 
 def P(t):
-    """Raw device/host pointer of a tensor (None -> NULL)."""
+    """Raw device/host pointer of a tensor as an int (None -> NULL)."""
     if t is None:
         return None
-    return C.c_void_p(t.data_ptr())
+    return t.data_ptr()
 
 
 def dt(t):
