@@ -763,6 +763,122 @@ __global__ void k_axpby(int64_t len, double a, const X *__restrict__ x, double b
   }
 }
 
+// ---- delayed CGS2 (DCGS2) Lanczos step ------------------------------------------------
+// Two reads of V[0..i] per step instead of CGS2's three: the second projection of the
+// provisional vector ~v_i = V[i] (orthogonalised once, in step i - 1) is taken in step i's
+// read together with the first projection of u = Op ~v_i, and one update pass then both
+// corrects V[i] and orthogonalises u (Swirydowicz, Langou, Ananthan, Yang, Thomas 2020;
+// Bielich, Langou, Thomas, Swirydowicz, Yamazaki, Boman 2022). The operator correction uses
+// the projected matrix T: Op V[0..i-1] = V[0..i] T[0..i, 0..i-1].
+constexpr int kDcgsMaxJ = 256;
+
+// part[p*2J + j] = V_j . u (j < J), part[p*2J + J + j] = V_j . w (j < J), w = V[J-1]
+template <int JT>
+__global__ void __launch_bounds__(256) k_dual_dot_part(int64_t len, int J, const double *__restrict__ V, int64_t ld,
+                                                        const double *__restrict__ u, const double *__restrict__ w,
+                                                        double *__restrict__ part) {
+  const int j0 = blockIdx.y * JT;
+  double a[JT], b[JT];
+#pragma unroll
+  for (int t = 0; t < JT; ++t) a[t] = b[t] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    const double ui = u[i], wi = w[i];
+#pragma unroll
+    for (int t = 0; t < JT; ++t)
+      if (j0 + t < J) {
+        const double v = V[(int64_t)(j0 + t) * ld + i];
+        a[t] += v * ui;
+        b[t] += v * wi;
+      }
+  }
+  __shared__ double red[2 * JT][4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int t = 0; t < JT; ++t) {
+    double x = a[t], y = b[t];
+    for (int off = 32; off > 0; off >>= 1) {
+      x += __shfl_xor(x, off, 64);
+      y += __shfl_xor(y, off, 64);
+    }
+    if (lane == 0) {
+      red[t][wv] = x;
+      red[JT + t][wv] = y;
+    }
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < 2 * JT) {
+    const int t = threadIdx.x % JT, which = threadIdx.x / JT;
+    if (j0 + t < J)
+      part[(int64_t)blockIdx.x * 2 * J + which * J + j0 + t] =
+          ((red[threadIdx.x][0] + red[threadIdx.x][1]) + (red[threadIdx.x][2] + red[threadIdx.x][3]));
+  }
+}
+
+__device__ __forceinline__ double block_sum256(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// One workgroup. sg = (s, g): s_j = V_j . u (j <= i), g_l = V_l . ~v_i (l < i), i = J - 1,
+// u = A ~v_i for the provisional basis vector ~v_i. The corrected vector is
+// v_i = ~v_i - sum_l g_l V_l (norm 1 + O(g^2)). Because A ~v_i - A v_i = sum_l g_l A V_l lies in
+// span(V_0..V_i) for a Krylov basis, projecting u off that span gives A v_i's residual:
+//   u' = u - sum_{j<i} s_j V_j - (v_i . u) v_i,  v_i . u = s_i - g . s,
+// and alpha_i = v_i . A v_i = v_i . u - sum_l g_l (v_i . A V_l) = s_i - g . s - beta_{i-1} g_{i-1}
+// (v_i . A V_l vanishes for l < i - 1, also for the Ritz vectors of a thick restart).
+// coef[i] = v_i . u, coef[j] = s_j - coef[i] g_j (j < i), coef[J + l] = g_l.
+__global__ void __launch_bounds__(256) k_dcgs2_coef(int J, const double *__restrict__ sg,
+                                                     const double *__restrict__ beta, double *__restrict__ coef,
+                                                     double *__restrict__ alpha_i) {
+  __shared__ double red[4];
+  const int i = J - 1;
+  const double *s = sg, *g = sg + J;
+  const int tid = threadIdx.x;
+  double p = 0.0;
+  for (int l = tid; l < i; l += blockDim.x) p += g[l] * s[l];
+  const double ci = s[i] - block_sum256(p, red);
+  for (int j = tid; j < i; j += blockDim.x) {
+    coef[j] = s[j] - ci * g[j];
+    coef[J + j] = g[j];
+  }
+  if (tid == 0) {
+    coef[i] = ci;
+    *alpha_i = i > 0 ? ci - beta[i - 1] * g[i - 1] : ci;
+  }
+}
+
+// u -= sum_{j<=i} coef[j] V_j and V[i] -= sum_{l<i} coef[J+l] V_l in one read of V[0..i];
+// part[block] = the block's sum of u'^2.
+__global__ void __launch_bounds__(256) k_dcgs2_update(int64_t len, int J, double *__restrict__ V, int64_t ld,
+                                                       const double *__restrict__ coef, double *__restrict__ u,
+                                                       double *__restrict__ part) {
+  __shared__ double ca[kDcgsMaxJ], cg[kDcgsMaxJ];
+  __shared__ double red[4];
+  const int i = J - 1;
+  for (int t = threadIdx.x; t <= i; t += blockDim.x) ca[t] = coef[t];
+  for (int t = threadIdx.x; t < i; t += blockDim.x) cg[t] = coef[J + t];
+  __syncthreads();
+  double ss = 0.0;
+  for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < len; d += (int64_t)gridDim.x * blockDim.x) {
+    double su = 0.0, sv = 0.0;
+    for (int j = 0; j < i; ++j) {
+      const double v = V[(int64_t)j * ld + d];
+      su += ca[j] * v;
+      sv += cg[j] * v;
+    }
+    const double vi = V[(int64_t)i * ld + d];
+    const double un = u[d] - (su + ca[i] * vi);
+    u[d] = un;
+    V[(int64_t)i * ld + d] = vi - sv;
+    ss += un * un;
+  }
+  const double tot = block_sum256(ss, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
 int grid_for(int64_t len, int bs, int cap) {
   int64_t g = (len + bs - 1) / bs;
   if (g > cap) g = cap;
@@ -1106,6 +1222,38 @@ inch_status inch_anm_matfree_cl_apply(int64_t n, const double *xyz, int64_t ncl,
   dim3 grid((unsigned)ncl, (unsigned)((nrhs + 3) / 4));
   k_matfree_clm<true><<<grid, 64, 0, st>>>(n, xyz, ncl, cluster_atoms, cl_ptr, base, entries, plus_i, x, y, nrhs);
   HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+int64_t inch_dcgs2_work_size(int32_t J) { return (int64_t)kRedBlocks * (2 * (int64_t)J + 1) + 4 * (int64_t)J; }
+
+inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
+                            const double *beta, double *beta_out, double *vnext, double *work, void *stream) {
+  INCH_REQUIRE(len > 0 && J >= 1 && J <= kDcgsMaxJ && V && u && alpha && beta_out && work && ld >= len,
+               "inch_dcgs2_step: bad args (J must be 1..256)");
+  INCH_REQUIRE(J == 1 || beta, "inch_dcgs2_step: beta needed for J > 1");
+  hipStream_t st = (hipStream_t)stream;
+  constexpr int JT = 8;
+  const int gx = grid_for(len, 256, kRedBlocks);
+  double *part = work;
+  double *sg = part + (int64_t)gx * 2 * J;
+  double *coef = sg + 2 * J;
+  double *npart = coef + 2 * J;
+  dim3 grid(gx, (J + JT - 1) / JT);
+  k_dual_dot_part<JT><<<grid, 256, 0, st>>>(len, J, V, ld, u, V + (int64_t)(J - 1) * ld, part);
+  HIP_TRY(hipGetLastError());
+  k_reduce_parts<<<2 * J, 256, 0, st>>>(gx, 2 * J, part, sg, 0);
+  HIP_TRY(hipGetLastError());
+  k_dcgs2_coef<<<1, 256, 0, st>>>(J, sg, beta, coef, alpha + (J - 1));
+  HIP_TRY(hipGetLastError());
+  k_dcgs2_update<<<gx, 256, 0, st>>>(len, J, V, ld, coef, u, npart);
+  HIP_TRY(hipGetLastError());
+  k_reduce_parts<<<1, 256, 0, st>>>(gx, 1, npart, beta_out, 1);
+  HIP_TRY(hipGetLastError());
+  if (vnext) {
+    k_scale_div<<<grid_for(len, 256, 4096), 256, 0, st>>>(len, u, beta_out, vnext, nullptr);
+    HIP_TRY(hipGetLastError());
+  }
   return INCH_OK;
 }
 

@@ -483,6 +483,28 @@ class CGS2Workspace:
         self.buf = torch.empty(int(lib().inch_cgs2_work_size(self.J)), dtype=torch.float64, device=device)
 
 
+class DCGS2Workspace:
+    def __init__(self, J, device):
+        from ._lib import lib
+        self.J = int(J)
+        self.buf = torch.empty(int(lib().inch_dcgs2_work_size(self.J)), dtype=torch.float64, device=device)
+
+
+def dcgs2_step(V, J, u, alpha, beta, beta_out, vnext=None, ws=None, stream=None):
+    """One Lanczos orthogonalisation step with delayed reorthogonalisation (inch_dcgs2_step):
+    i = J - 1, u = Op V[i] for the provisional V[i]; corrects V[i] against V[:i], projects u
+    off the corrected V[:J], writes alpha[i] (using beta[i - 1]), beta_out = |u| and
+    vnext = u / |u|."""
+    if J > CGS2_MAX_J or J < 1:
+        raise ValueError("dcgs2_step: J must be in 1..256")
+    if V.shape[0] < J or V.shape[1] != u.numel() or alpha.numel() < J or (J > 1 and beta.numel() < J - 1):
+        raise ValueError("dcgs2_step: shape mismatch")
+    if ws is None or ws.J < J:
+        ws = DCGS2Workspace(J, u.device)
+    call("inch_dcgs2_step", u.numel(), int(J), P(V), V.stride(0), P(u), P(alpha), P(beta), P(beta_out), P(vnext),
+         P(ws.buf), stream_ptr(stream))
+
+
 def cgs2_reorth(V, J, h1, u, beta_out, vnext=None, ws=None, stream=None):
     """u' = u - V[:J]^T h1; beta = ||u'||; u = u' - V[:J]^T (V[:J] u'); vnext = u / beta."""
     if J > CGS2_MAX_J or J < 1:

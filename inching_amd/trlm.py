@@ -61,7 +61,7 @@ class TRLM:
     beta while segment s + 1 runs; the cycle ends after segment s + 1 once e_s <= stop_est
     or the geometric extrapolation e_s^2 / e_(s-1) to segment s + 1 is <= 0.3 stop_est."""
 
-    def __init__(self, op, k, factor=2, ncv=None, use_graph=True):
+    def __init__(self, op, k, factor=2, ncv=None, use_graph=True, dcgs2=True):
         n = op.n
         if k >= n:
             raise AssertionError("ABORTED. k must be smaller than n")
@@ -80,6 +80,10 @@ class TRLM:
         self.betak_d = torch.empty(self.k, dtype=torch.float64, device=dev)
         self.ws = ops.Workspace(self.ncv + 1, dev)
         self.cgs_ws = ops.CGS2Workspace(min(self.ncv, ops.CGS2_MAX_J), dev)
+        # delayed reorthogonalisation (two reads of V per step) when the whole basis fits it
+        self.dcgs2 = bool(dcgs2) and self.ncv <= ops.CGS2_MAX_J
+        if self.dcgs2:
+            self.dcgs_ws = ops.DCGS2Workspace(self.ncv, dev)
         self.use_graph = bool(use_graph) and op.timer is None
         self.graphs = {}
         self.stop_est = None    # residual-estimate threshold that ends a cycle early
@@ -89,11 +93,17 @@ class TRLM:
         self._bh = torch.empty(self.ncv, dtype=torch.float64, pin_memory=True)
 
     # -- Lanczos steps i_start .. i_end-1: every step leaves its residual in u and, while the
-    # basis has room (i + 1 < ncv), the next basis vector u / beta_i in V[i + 1]
+    # basis has room (i + 1 < ncv), the next basis vector u / beta_i in V[i + 1]. With DCGS2
+    # that vector is provisional until the next step corrects it (see _finish_cycle).
     def lanczos(self, i_start, i_end):
         V, u, h, ws = self.V, self.u, self.h, self.ws
         for i in range(i_start, i_end):
             self.op.apply(V[i], u)
+            if self.dcgs2:
+                last = i >= self.ncv - 1
+                ops.dcgs2_step(V, i + 1, u, self.alpha, self.beta, self.beta[i:i + 1],
+                               vnext=None if last else V[i + 1], ws=self.dcgs_ws)
+                continue
             ops.multidot(V, i + 1, u, h, ws)
             self.alpha[i:i + 1].copy_(h[i:i + 1])
             last = i >= self.ncv - 1
@@ -178,6 +188,13 @@ class TRLM:
             pending = (hi, ev)
         return self.ncv
 
+    def _finish_cycle(self, m):
+        """With DCGS2 the last residual u has had one orthogonalisation pass (its second one
+        would come in the next step): give it that pass against V[:m] before the restart."""
+        if self.dcgs2:
+            ops.multidot(self.V, m, self.u, self.h, self.ws)
+            ops.gemv_update(self.V, m, self.h, self.u, 1.0)
+
     def _restart_rr(self, alpha_h, beta_h, beta_k, filtered):
         S, W = select_ritz(alpha_h, beta_h, beta_k, self.k, filtered)
         Wd = torch.as_tensor(np.ascontiguousarray(W), device=self.V.device)
@@ -206,6 +223,7 @@ class TRLM:
             self.u.copy_(torch.as_tensor(v0, dtype=torch.float64, device=self.V.device).reshape(-1))
         self.V[0].copy_(self.u / torch.linalg.norm(self.u))
         self.lanczos(0, ncv)
+        self._finish_cycle(ncv)
         self.steps = ncv
         alpha_h = self.alpha.cpu().numpy().copy()
         beta_h = self.beta.cpu().numpy().copy()
@@ -221,6 +239,7 @@ class TRLM:
                 break
             self.betak_d.copy_(torch.as_tensor(beta_k, device=self.V.device))
             m = self._run_cycle(S, beta_k, filtered)
+            self._finish_cycle(m)
             self.steps += m - k
             alpha_h = self.alpha[:m].cpu().numpy().copy()
             beta_h = self.beta[:m].cpu().numpy().copy()

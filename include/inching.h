@@ -273,6 +273,18 @@ inch_status inch_residual_norms(int64_t len, int32_t k, const double *X, int64_t
 int64_t inch_cgs2_work_size(int32_t J);
 inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld, const double *h1, double *u,
                              double *vnext, double *beta_out, double *work, void *stream);
+/* Lanczos orthogonalisation with delayed reorthogonalisation (DCGS2; no reference
+ * counterpart as one call: it replaces the CGS2 sequence of inch_multidot +
+ * inch_cgs2_reorth, TRLM/T1.py:465-520, with two reads of V instead of three). i = J - 1;
+ * V[i] is the provisional vector of the previous step and u = Op V[i]. One read of V[0..i]
+ * gives s = V[0..i] u and g = V[0..i-1] V[i]; a second read sets V[i] -= V[0..i-1] g and
+ * projects u off the corrected basis (u - sum_{j<i} s_j V_j - (s_i - g.s) v_i).
+ * alpha[i] = s_i - g.s - beta[i-1] g[i-1] (the only operator term that survives in a Krylov
+ * basis, thick restarts included). Writes alpha[i], *beta_out = |u| and vnext = u / |u|
+ * (when non-NULL). work: inch_dcgs2_work_size(J) doubles. J <= 256. */
+int64_t inch_dcgs2_work_size(int32_t J);
+inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
+                            const double *beta, double *beta_out, double *vnext, double *work, void *stream);
 
 /* out[q*ld_out + i] = sum_r W[r*k + q] * V[r*ld + i]  for q < k, r < m
  * (W float64 [m, k] row-major on device; out must not alias V). */

@@ -323,6 +323,62 @@ def test_cgs2_reorth_fused(cuda, J, n):
     assert np.max(np.abs(V @ Vd[J].cpu().numpy())) < 1e-13
 
 
+def _projected_system(n, i, kk, rng):
+    """Orthonormal Q [n, i + 2] and a symmetric A whose restriction to span(Q[:, :i + 1]) is the
+    thick-restart projected matrix T (Ritz values theta on rows < kk, arrowhead couplings
+    betak to row kk, then tridiagonal alpha/beta) with A Q_i = sum_m T[m, i] Q_m + beta_i Q_{i+1};
+    the orthogonal complement carries a random symmetric block."""
+    Qf = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    m = i + 2
+    alpha = rng.standard_normal(m)
+    beta = rng.uniform(0.5, 2.0, m)
+    theta = rng.standard_normal(kk)
+    betak = rng.uniform(-1.0, 1.0, kk)
+    T = np.zeros((m, m))
+    for j in range(kk):
+        T[j, j] = theta[j]
+        T[j, kk] = T[kk, j] = betak[j]
+    for j in range(kk, m):
+        T[j, j] = alpha[j]
+        if j + 1 < m:
+            T[j, j + 1] = T[j + 1, j] = beta[j]
+    R = rng.standard_normal((n - m, n - m))
+    Af = np.zeros((n, n))
+    Af[:m, :m] = T
+    Af[m:, m:] = 0.5 * (R + R.T)
+    A = Qf @ Af @ Qf.T
+    return Qf[:, :m], A, alpha, beta, theta, betak, T
+
+
+@pytest.mark.parametrize("i,kk,eps", [(0, 0, 1e-4), (1, 0, 1e-4), (20, 0, 1e-4), (20, 6, 1e-4), (20, 6, 0.0),
+                                      (40, 39, 1e-4)])
+def test_dcgs2_step_first_order_exact(cuda, i, kk, eps):
+    """Delayed CGS2 on a provisional basis vector ~v_i = Q_i + sum_l eps_l Q_l (normalised), with
+    kk restart (Ritz) rows in the projected matrix: the step must return the corrected
+    V[i] = Q_i, alpha_i = T[i, i] and the residual beta_i Q_{i+1} up to O(eps^2); using the
+    uncorrected V[i] anywhere, or dropping alpha's operator term, leaves O(eps) = 1e-4."""
+    from inching_amd import ops
+    print("case: dcgs2 i=%d kk=%d eps=%g" % (i, kk, eps), flush=True)
+    rng = np.random.default_rng(100 * i + kk)
+    n = 300
+    Q, A, alpha, beta, theta, betak, T = _projected_system(n, i, kk, rng)
+    vt = Q[:, i] + Q[:, :i] @ (eps * rng.standard_normal(i))
+    vt /= np.linalg.norm(vt)
+    V = np.concatenate([Q[:, :i].T, vt[None, :], np.zeros((1, n))])
+    Vd, ud = _dev(V), _dev(A @ vt)
+    al = _dev(np.where(np.arange(i + 2) < i, alpha[:i + 2], np.nan))      # alpha_i is the output
+    bout = torch.empty(1, dtype=torch.float64, device="cuda")
+    ops.dcgs2_step(Vd, i + 1, ud, al, _dev(beta), bout, vnext=Vd[i + 1])
+    bar = 2e-6 if eps else 1e-12           # correct: <= 5e-7 at eps = 1e-4; each mistake: >= 6e-5
+    Vo = Vd.cpu().numpy()
+    assert np.array_equal(Vo[:i], Q[:, :i].T)                           # V[:i] untouched
+    assert np.linalg.norm(Vo[i] - Q[:, i]) < bar
+    assert abs(al[i].item() - T[i, i]) < bar
+    assert abs(bout.item() - beta[i]) < bar
+    assert np.linalg.norm(ud.cpu().numpy() - beta[i] * Q[:, i + 1]) < bar
+    assert np.linalg.norm(Vo[i + 1] - Q[:, i + 1]) < bar
+
+
 def test_spmv_col16_bitwise_equal(mat):
     """16-bit column offsets (row base + offset) give bitwise the int32-column result for every
     lane count and dtype pair; a row spanning more than 16 bits is rejected."""
