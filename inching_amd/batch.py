@@ -62,6 +62,19 @@ class SegOps:
         self.seg = seg
         self.maxJ = int(maxJ)
         self.wk = seg.work(maxJ, device)
+        self.dJ = min(self.maxJ, 256)
+        nw = call("inch_seg_dcgs2_work_size", seg.nchunk, seg.S, self.dJ)
+        self.wk2 = torch.empty(int(nw), dtype=torch.float64, device=device)
+
+    def dcgs2(self, V, J, u, alpha_i, beta_prev, beta_out):
+        """Delayed CGS2 step per segment (inch_seg_dcgs2_step): corrects the provisional V[J-1]
+        against V[:J-1], projects u = Op V[J-1] off the corrected V[:J]; writes alpha_i[s]
+        (using beta_prev[s]) and beta_out[s] = |u_s| (two reads of V)."""
+        s = self.seg
+        if J > self.dJ:
+            raise ValueError("dcgs2: workspace holds J <= %d" % self.dJ)
+        call("inch_seg_dcgs2_step", s.nchunk, s.S, P(s.cbeg), P(s.cend), P(s.cseg), P(s.sptr), int(J), P(V),
+             V.stride(0), P(u), P(alpha_i), P(beta_prev), P(beta_out), P(self.wk2), SP())
 
     def cgs_b(self, V, J, h, hld, u, h2, nrm):
         """u -= V[:J] h per segment; h2[s] = (V[:J] u, |u|^2) over segment s; nrm[s] = |u| (one
@@ -152,7 +165,7 @@ class BatchResult:
 
 
 def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
-                factor=2, seed=0, use_graph=True, verbose=False, pol=None):
+                factor=2, seed=0, use_graph=True, verbose=False, pol=None, dcgs2=True):
     """Lowest k non-trivial modes of every structure (list of [N_s, 3] coordinate arrays, A).
 
     Returns BatchResult with per-structure eigenvalues [S, k] (PlusI included), eigenvectors
@@ -174,7 +187,7 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
         raise RuntimeError("structures are not contiguous after ordering (are they closer than the cutoff?)")
     return solve_segments(A, seg_atoms, k, seg_struct=seg_struct, precision=precision, tol_rel=tol_rel, shift=shift,
                           maxiter=maxiter, factor=factor, seed=seed, use_graph=use_graph, verbose=verbose, t0=t0,
-                          pol=pol)
+                          pol=pol, dcgs2=dcgs2)
 
 
 def components(A):
@@ -188,7 +201,7 @@ def components(A):
 
 
 def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
-                   factor=2, seed=0, use_graph=True, verbose=False, t0=None, pol=None, ptol=1e-10):
+                   factor=2, seed=0, use_graph=True, verbose=False, t0=None, pol=None, ptol=1e-10, dcgs2=True):
     """Lockstep thick-restart Lanczos over the contiguous atom segments of a block-diagonal
     A: k modes per segment, per-segment rigid-body Hotelling deflation, optional common
     Chebyshev filter ``pol`` (largest k of p(A) per segment, then Rayleigh quotients)."""
@@ -208,6 +221,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     if ncv <= k + 1:
         raise AssertionError("ABORTED. k must be smaller than every structure's dof count minus the rigid modes")
     so = SegOps(seg, max(ncv, 6) + 1, dev)
+    dcgs2 = bool(dcgs2) and ncv <= so.dJ
     Q = segmented_rigid_basis(A.xyz, seg_atoms, so)
     op = Operator(A, precision, pol=pol)
     filtered = pol is not None
@@ -233,6 +247,12 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
         for i in range(i0, i1):
             apply(V[i], u)
             J = i + 1
+            if dcgs2:
+                # delayed CGS2: V[i] is corrected here, its successor at the next step
+                so.dcgs2(V, J, u, alpha[i], beta[i - 1] if i > 0 else None, beta[i])
+                if i < i1 - 1:
+                    so.scale(u, beta[i], V[i + 1])
+                continue
             so.multidot(V, J, u, h)
             alpha[i].copy_(h[:S * J].view(S, J)[:, i])
             # CGS2: update + second projection in one read of V (beta from the first update),
@@ -252,6 +272,13 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
         so.update(V, k, betak_d, k, 1.0, u, nrm=beta[k])
         so.scale(u, beta[k], V[k + 1])
         lanczos(k + 1, ncv)
+        finish()
+
+    def finish():
+        # with delayed CGS2 the last residual has had one pass: give it the second
+        if dcgs2:
+            so.multidot(V, ncv, u, h)
+            so.update(V, ncv, h, ncv, 1.0, u)
 
     def restart(a, b, bk):
         T = np.zeros((S, ncv, ncv))
@@ -279,6 +306,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     so.update(V, 0, None, 1, 0.0, u, nrm=scal)
     so.scale(u, scal, V[0])
     lanczos(0, ncv)
+    finish()
     a, b = alpha.cpu().numpy(), beta.cpu().numpy()
     Sel, bk = restart(a, b, None)
     lam_max = None

@@ -145,6 +145,101 @@ __global__ void __launch_bounds__(256) k_seg_cgs_b(const int64_t *__restrict__ c
   if (tid == 0) part[(int64_t)c * (J + 1) + J] = ss;
 }
 
+// Delayed CGS2 step, per segment (the segmented form of inch_dcgs2_step): i = J - 1, V[i] is
+// the provisional vector, u = Op V[i].
+// part[c*2J + j] = V_j . u and part[c*2J + J + j] = V_j . w over chunk c (w = V[J-1]).
+__global__ void __launch_bounds__(256) k_seg_dual_dot_part(const int64_t *__restrict__ cbeg,
+                                                            const int64_t *__restrict__ cend, int J,
+                                                            const double *__restrict__ V, int64_t ld,
+                                                            const double *__restrict__ u,
+                                                            const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int64_t b = cbeg[c], e = cend[c];
+  for (int j0 = 0; j0 < J; j0 += 8) {
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+      const double ui = u[i], wi = w[i];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (j0 + t < J) {
+          const double v = V[(int64_t)(j0 + t) * ld + i];
+          a[t] += v * ui;
+          g[t] += v * wi;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const double sa = block_sum(a[t], red);
+      const double sg = block_sum(g[t], red);
+      if (threadIdx.x == 0 && j0 + t < J) {
+        part[(int64_t)c * 2 * J + j0 + t] = sa;
+        part[(int64_t)c * 2 * J + J + j0 + t] = sg;
+      }
+    }
+  }
+}
+
+// One wave per segment s: sg[s] = (s_0..s_i, g_0..g_{i-1}, -). With v_i = ~v_i - sum g_l V_l:
+// coef[s, i] = v_i . u = s_i - g . s, coef[s, j] = s_j - coef[s, i] g_j (j < i),
+// coef[s, J + l] = g_l and alpha_i[s] = coef[s, i] - beta_prev[s] g_{i-1} (the one operator
+// term left in a Krylov basis).
+__global__ void __launch_bounds__(64) k_seg_dcgs2_coef(int J, const double *__restrict__ sg,
+                                                        const double *__restrict__ beta_prev,
+                                                        double *__restrict__ coef, double *__restrict__ alpha_i) {
+  const int s = blockIdx.x, lane = threadIdx.x;
+  const int i = J - 1;
+  const double *sv = sg + (int64_t)s * 2 * J, *g = sv + J;
+  double *cf = coef + (int64_t)s * 2 * J;
+  double p = 0.0;
+  for (int l = lane; l < i; l += 64) p += g[l] * sv[l];
+  for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off, 64);
+  const double ci = sv[i] - p;
+  for (int j = lane; j < i; j += 64) {
+    cf[j] = sv[j] - ci * g[j];
+    cf[J + j] = g[j];
+  }
+  if (lane == 0) {
+    cf[i] = ci;
+    alpha_i[s] = i > 0 ? ci - beta_prev[s] * g[i - 1] : ci;
+  }
+}
+
+// u -= sum_{j<=i} coef[s, j] V_j and V[i] -= sum_{l<i} coef[s, J + l] V_l over chunk c of
+// segment s, in one read of V[0..i]; part[c] = |u'|^2 over the chunk.
+__global__ void __launch_bounds__(256) k_seg_dcgs2_update(const int64_t *__restrict__ cbeg,
+                                                           const int64_t *__restrict__ cend,
+                                                           const int32_t *__restrict__ cseg, int J,
+                                                           double *__restrict__ V, int64_t ld,
+                                                           const double *__restrict__ coef, double *__restrict__ u,
+                                                           double *__restrict__ part) {
+  __shared__ double ca[kSegMaxJ / 2], cg[kSegMaxJ / 2];
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  const int i = J - 1;
+  const double *cf = coef + (int64_t)s * 2 * J;
+  for (int t = threadIdx.x; t <= i; t += blockDim.x) ca[t] = cf[t];
+  for (int t = threadIdx.x; t < i; t += blockDim.x) cg[t] = cf[J + t];
+  __syncthreads();
+  double ss = 0.0;
+  for (int64_t x = cbeg[c] + threadIdx.x; x < cend[c]; x += blockDim.x) {
+    double su = 0.0, sv = 0.0;
+    for (int j = 0; j < i; ++j) {
+      const double v = V[(int64_t)j * ld + x];
+      su += ca[j] * v;
+      sv += cg[j] * v;
+    }
+    const double vi = V[(int64_t)i * ld + x];
+    const double un = u[x] - (su + ca[i] * vi);
+    u[x] = un;
+    V[(int64_t)i * ld + x] = vi - sv;
+    ss += un * un;
+  }
+  const double tot = block_sum(ss, red);
+  if (threadIdx.x == 0) part[c] = tot;
+}
+
 // dst[s] = sqrt(src[s * stride + off])
 __global__ void k_seg_sqrt_col(int S, const double *__restrict__ src, int stride, int off, double *__restrict__ dst) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -267,6 +362,37 @@ inch_status inch_seg_cgs_b(int32_t nchunk, int32_t S, const int64_t *cbeg, const
   k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, J + 1, sptr, work, h2, 0);
   HIP_TRY(hipGetLastError());
   k_seg_sqrt_col<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, h2, J + 1, J, nrm_out);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+int64_t inch_seg_dcgs2_work_size(int32_t nchunk, int32_t S, int32_t J) {
+  return (int64_t)nchunk * (2 * (int64_t)J + 1) + 4 * (int64_t)S * J;
+}
+
+inch_status inch_seg_dcgs2_step(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend,
+                                const int32_t *cseg, const int64_t *sptr, int32_t J, double *V, int64_t ld, double *u,
+                                double *alpha_i, const double *beta_prev, double *beta_out, double *work,
+                                void *stream) {
+  INCH_REQUIRE(nchunk > 0 && S > 0 && J > 0 && J <= kSegMaxJ / 2 && cbeg && cend && cseg && sptr && V && u &&
+                   alpha_i && beta_out && work,
+               "inch_seg_dcgs2_step: bad args (0 < J <= 256)");
+  INCH_REQUIRE(J == 1 || beta_prev, "inch_seg_dcgs2_step: beta_prev needed for J > 1");
+  hipStream_t st = (hipStream_t)stream;
+  double *part = work;
+  double *sg = part + (int64_t)nchunk * 2 * J;
+  double *coef = sg + (int64_t)S * 2 * J;
+  double *npart = coef + (int64_t)S * 2 * J;
+  k_seg_dual_dot_part<<<nchunk, 256, 0, st>>>(cbeg, cend, J, V, ld, u, V + (int64_t)(J - 1) * ld, part);
+  HIP_TRY(hipGetLastError());
+  const int64_t tot = (int64_t)S * 2 * J;
+  k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, 2 * J, sptr, part, sg, 0);
+  HIP_TRY(hipGetLastError());
+  k_seg_dcgs2_coef<<<S, 64, 0, st>>>(J, sg, beta_prev, coef, alpha_i);
+  HIP_TRY(hipGetLastError());
+  k_seg_dcgs2_update<<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, J, V, ld, coef, u, npart);
+  HIP_TRY(hipGetLastError());
+  k_seg_reduce<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, 1, sptr, npart, beta_out, 1);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

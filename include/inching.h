@@ -323,7 +323,6 @@ inch_status inch_seg_multidot(int32_t nchunk, int32_t S, const int64_t *cbeg, co
 inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
                             const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
                             double scale, double *u, double *nrm_out, double *work, void *stream);
-/* dst[i] = src[i] / div[s] (and dst2). */
 /* CGS2's update and second projection in one read of V, per segment: u -= V[:J] h (h of
  * each segment, row stride hld), then h2[s, j] = V[j] . u over segment s (j < J),
  * h2[s, J] = |u|^2 over s and nrm_out[s] = sqrt(h2[s, J]); h2 is [S, J + 1]. work holds
@@ -331,6 +330,17 @@ inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, cons
 inch_status inch_seg_cgs_b(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
                            const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
                            double *u, double *h2, double *nrm_out, double *work, void *stream);
+/* Delayed CGS2 Lanczos step per segment (the segmented inch_dcgs2_step; replaces the
+ * per-structure CGS2 of TRLM/T1.py:465-520 with two reads of V instead of three): i = J - 1,
+ * V[i] provisional, u = Op V[i]. Corrects V[i] -= V[:i] g per segment, projects u off the
+ * corrected V[:J], alpha_i[s] = v_i . u - beta_prev[s] g_{i-1}, beta_out[s] = |u_s|.
+ * work: inch_seg_dcgs2_work_size doubles. J <= 256. */
+int64_t inch_seg_dcgs2_work_size(int32_t nchunk, int32_t S, int32_t J);
+inch_status inch_seg_dcgs2_step(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend,
+                                const int32_t *cseg, const int64_t *sptr, int32_t J, double *V, int64_t ld, double *u,
+                                double *alpha_i, const double *beta_prev, double *beta_out, double *work,
+                                void *stream);
+/* dst[i] = src[i] / div[s] (and dst2). */
 inch_status inch_seg_scale(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
                            const double *src, const double *div, double *dst, double *dst2, void *stream);
 /* out[q*ld_out+i] = sum_r W[s][r][q] V[r*ld+i] (W [S, m, k], k <= 64, m*k <= 8192). */

@@ -3,7 +3,7 @@ import numpy as np
 import pytest
 
 from oracle import hessian as OH
-from tests.helpers import small_globule
+from tests.helpers import projected_system, small_globule
 
 pytestmark = pytest.mark.gpu
 
@@ -83,6 +83,62 @@ def test_segmented_fused_cgs_pass(cuda, J):
         assert np.allclose(got[s, :J], V[:, sl] @ uref[sl], rtol=1e-10, atol=1e-8)
         assert np.isclose(got[s, J], uref[sl] @ uref[sl], rtol=1e-12)
         assert np.isclose(nrm.cpu().numpy()[s], np.linalg.norm(uref[sl]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("i", [0, 1, 19, 59])
+def test_segmented_dcgs2_step_first_order_exact(cuda, i):
+    """Segmented delayed CGS2: every segment holds its own basis with a known projected matrix
+    (arrowhead rows for some) and a provisional vector perturbed by 1e-4; per segment the
+    corrected V[i], alpha_i, beta_i and the residual must agree to < 1e-6, where a wrong
+    or missing correction leaves O(1e-4). Segments are cut into several chunks."""
+    import torch
+    from inching_amd.batch import SegOps, Segments
+    print("case: seg dcgs2 i=%d" % i, flush=True)
+    dofs, kks = [300, 257, 411, 1000], [0, 0, min(6, max(i - 1, 0)), max(i - 2, 0)]
+    seg_ptr = np.concatenate([[0], np.cumsum(dofs)])
+    n, S = int(seg_ptr[-1]), len(dofs)
+    seg = Segments(seg_ptr, torch.device("cuda"), chunk=128)
+    so = SegOps(seg, i + 2, torch.device("cuda"))
+    V = np.zeros((i + 2, n))
+    u = np.zeros(n)
+    want = []
+    alpha = np.full((i + 2, S), np.nan)
+    beta = np.zeros((i + 2, S))
+    for s, (m, kk) in enumerate(zip(dofs, kks)):
+        rng = np.random.default_rng(1000 * i + s)
+        Q, A, al, be, _, _, T = projected_system(m, i, kk, rng)
+        vt = Q[:, i] + Q[:, :i] @ (1e-4 * rng.standard_normal(i))
+        vt /= np.linalg.norm(vt)
+        sl = slice(seg_ptr[s], seg_ptr[s + 1])
+        V[:i, sl] = Q[:, :i].T
+        V[i, sl] = vt
+        u[sl] = A @ vt
+        alpha[:i, s] = al[:i]
+        beta[:, s] = be[:i + 2]
+        want.append((sl, Q, T[i, i], be[i]))
+    Vd, ud = torch.as_tensor(V, device="cuda"), torch.as_tensor(u, device="cuda")
+    ad, bd = torch.as_tensor(alpha, device="cuda"), torch.as_tensor(beta, device="cuda")
+    bout = torch.empty(S, dtype=torch.float64, device="cuda")
+    so.dcgs2(Vd, i + 1, ud, ad[i], bd[i - 1] if i > 0 else None, bout)
+    Vo, uo, ao, bo = Vd.cpu().numpy(), ud.cpu().numpy(), ad.cpu().numpy(), bout.cpu().numpy()
+    assert np.array_equal(Vo[:i], V[:i])
+    for s, (sl, Q, a_ref, b_ref) in enumerate(want):
+        assert np.linalg.norm(Vo[i, sl] - Q[:, i]) < 2e-6, s
+        assert abs(ao[i, s] - a_ref) < 2e-6, s
+        assert abs(bo[s] - b_ref) < 2e-6, s
+        assert np.linalg.norm(uo[sl] - b_ref * Q[:, i + 1]) < 2e-6, s
+
+
+def test_batched_solve_dcgs2_matches_cgs2(cuda):
+    """The lockstep solve with delayed CGS2 and with fused CGS2 agree on every eigenvalue."""
+    from inching_amd.batch import solve_batch
+    print("case: batch dcgs2 vs cgs2", flush=True)
+    structs = [small_globule(n, seed=70 + i) for i, n in enumerate([380, 520, 450])]
+    r1 = solve_batch(structs, k=8, tol_rel=1e-10, maxiter=400)
+    r2 = solve_batch(structs, k=8, tol_rel=1e-10, maxiter=400, dcgs2=False)
+    assert r1.converged == r2.converged == len(structs)
+    assert np.allclose(r1.eigval, r2.eigval, rtol=0, atol=1e-9)
+    assert r1.rel_res.max() < 1e-9
 
 
 @pytest.mark.parametrize("filtered,per_component", [(False, 12), (True, 12), (True, 4)])

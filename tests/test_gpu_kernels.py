@@ -13,7 +13,7 @@ import torch
 from oracle import geometry as OG
 from oracle import hessian as OH
 from oracle import rcm as OR
-from tests.helpers import small_globule
+from tests.helpers import projected_system, small_globule
 
 pytestmark = pytest.mark.gpu
 
@@ -323,33 +323,6 @@ def test_cgs2_reorth_fused(cuda, J, n):
     assert np.max(np.abs(V @ Vd[J].cpu().numpy())) < 1e-13
 
 
-def _projected_system(n, i, kk, rng):
-    """Orthonormal Q [n, i + 2] and a symmetric A whose restriction to span(Q[:, :i + 1]) is the
-    thick-restart projected matrix T (Ritz values theta on rows < kk, arrowhead couplings
-    betak to row kk, then tridiagonal alpha/beta) with A Q_i = sum_m T[m, i] Q_m + beta_i Q_{i+1};
-    the orthogonal complement carries a random symmetric block."""
-    Qf = np.linalg.qr(rng.standard_normal((n, n)))[0]
-    m = i + 2
-    alpha = rng.standard_normal(m)
-    beta = rng.uniform(0.5, 2.0, m)
-    theta = rng.standard_normal(kk)
-    betak = rng.uniform(-1.0, 1.0, kk)
-    T = np.zeros((m, m))
-    for j in range(kk):
-        T[j, j] = theta[j]
-        T[j, kk] = T[kk, j] = betak[j]
-    for j in range(kk, m):
-        T[j, j] = alpha[j]
-        if j + 1 < m:
-            T[j, j + 1] = T[j + 1, j] = beta[j]
-    R = rng.standard_normal((n - m, n - m))
-    Af = np.zeros((n, n))
-    Af[:m, :m] = T
-    Af[m:, m:] = 0.5 * (R + R.T)
-    A = Qf @ Af @ Qf.T
-    return Qf[:, :m], A, alpha, beta, theta, betak, T
-
-
 @pytest.mark.parametrize("i,kk,eps", [(0, 0, 1e-4), (1, 0, 1e-4), (20, 0, 1e-4), (20, 6, 1e-4), (20, 6, 0.0),
                                       (40, 39, 1e-4)])
 def test_dcgs2_step_first_order_exact(cuda, i, kk, eps):
@@ -361,7 +334,7 @@ def test_dcgs2_step_first_order_exact(cuda, i, kk, eps):
     print("case: dcgs2 i=%d kk=%d eps=%g" % (i, kk, eps), flush=True)
     rng = np.random.default_rng(100 * i + kk)
     n = 300
-    Q, A, alpha, beta, theta, betak, T = _projected_system(n, i, kk, rng)
+    Q, A, alpha, beta, theta, betak, T = projected_system(n, i, kk, rng)
     vt = Q[:, i] + Q[:, :i] @ (eps * rng.standard_normal(i))
     vt /= np.linalg.norm(vt)
     V = np.concatenate([Q[:, :i].T, vt[None, :], np.zeros((1, n))])
