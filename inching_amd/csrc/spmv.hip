@@ -851,16 +851,23 @@ __global__ void __launch_bounds__(256) k_dcgs2_coef(int J, const double *__restr
 }
 
 // u -= sum_{j<=i} coef[j] V_j and V[i] -= sum_{l<i} coef[J+l] V_l in one read of V[0..i];
-// part[block] = the block's sum of u'^2.
+// part[block] = the block's sum of u'^2. LDS: coefficients staged in LDS (J <= 256);
+// otherwise read from global memory (wave-uniform loads).
+template <bool LDS>
 __global__ void __launch_bounds__(256) k_dcgs2_update(int64_t len, int J, double *__restrict__ V, int64_t ld,
                                                        const double *__restrict__ coef, double *__restrict__ u,
                                                        double *__restrict__ part) {
-  __shared__ double ca[kDcgsMaxJ], cg[kDcgsMaxJ];
+  __shared__ double cs[LDS ? 2 * kDcgsMaxJ : 1];
   __shared__ double red[4];
   const int i = J - 1;
-  for (int t = threadIdx.x; t <= i; t += blockDim.x) ca[t] = coef[t];
-  for (int t = threadIdx.x; t < i; t += blockDim.x) cg[t] = coef[J + t];
-  __syncthreads();
+  const double *ca = coef, *cg = coef + J;
+  if constexpr (LDS) {
+    for (int t = threadIdx.x; t <= i; t += blockDim.x) cs[t] = coef[t];
+    for (int t = threadIdx.x; t < i; t += blockDim.x) cs[kDcgsMaxJ + t] = coef[J + t];
+    __syncthreads();
+    ca = cs;
+    cg = cs + kDcgsMaxJ;
+  }
   double ss = 0.0;
   for (int64_t d = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; d < len; d += (int64_t)gridDim.x * blockDim.x) {
     double su = 0.0, sv = 0.0;
@@ -1229,8 +1236,7 @@ int64_t inch_dcgs2_work_size(int32_t J) { return (int64_t)kRedBlocks * (2 * (int
 
 inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
                             const double *beta, double *beta_out, double *vnext, double *work, void *stream) {
-  INCH_REQUIRE(len > 0 && J >= 1 && J <= kDcgsMaxJ && V && u && alpha && beta_out && work && ld >= len,
-               "inch_dcgs2_step: bad args (J must be 1..256)");
+  INCH_REQUIRE(len > 0 && J >= 1 && V && u && alpha && beta_out && work && ld >= len, "inch_dcgs2_step: bad args");
   INCH_REQUIRE(J == 1 || beta, "inch_dcgs2_step: beta needed for J > 1");
   hipStream_t st = (hipStream_t)stream;
   constexpr int JT = 8;
@@ -1246,7 +1252,10 @@ inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, doubl
   HIP_TRY(hipGetLastError());
   k_dcgs2_coef<<<1, 256, 0, st>>>(J, sg, beta, coef, alpha + (J - 1));
   HIP_TRY(hipGetLastError());
-  k_dcgs2_update<<<gx, 256, 0, st>>>(len, J, V, ld, coef, u, npart);
+  if (J <= kDcgsMaxJ)
+    k_dcgs2_update<true><<<gx, 256, 0, st>>>(len, J, V, ld, coef, u, npart);
+  else
+    k_dcgs2_update<false><<<gx, 256, 0, st>>>(len, J, V, ld, coef, u, npart);
   HIP_TRY(hipGetLastError());
   k_reduce_parts<<<1, 256, 0, st>>>(gx, 1, npart, beta_out, 1);
   HIP_TRY(hipGetLastError());

@@ -495,8 +495,8 @@ def dcgs2_step(V, J, u, alpha, beta, beta_out, vnext=None, ws=None, stream=None)
     i = J - 1, u = Op V[i] for the provisional V[i]; corrects V[i] against V[:i], projects u
     off the corrected V[:J], writes alpha[i] (using beta[i - 1]), beta_out = |u| and
     vnext = u / |u|."""
-    if J > CGS2_MAX_J or J < 1:
-        raise ValueError("dcgs2_step: J must be in 1..256")
+    if J < 1:
+        raise ValueError("dcgs2_step: J must be >= 1")
     if V.shape[0] < J or V.shape[1] != u.numel() or alpha.numel() < J or (J > 1 and beta.numel() < J - 1):
         raise ValueError("dcgs2_step: shape mismatch")
     if ws is None or ws.J < J:

@@ -80,8 +80,8 @@ class TRLM:
         self.betak_d = torch.empty(self.k, dtype=torch.float64, device=dev)
         self.ws = ops.Workspace(self.ncv + 1, dev)
         self.cgs_ws = ops.CGS2Workspace(min(self.ncv, ops.CGS2_MAX_J), dev)
-        # delayed reorthogonalisation (two reads of V per step) when the whole basis fits it
-        self.dcgs2 = bool(dcgs2) and self.ncv <= ops.CGS2_MAX_J
+        # delayed reorthogonalisation: two reads of V per step (the CGS2 forms take three)
+        self.dcgs2 = bool(dcgs2)
         if self.dcgs2:
             self.dcgs_ws = ops.DCGS2Workspace(self.ncv, dev)
         self.use_graph = bool(use_graph) and op.timer is None

@@ -281,7 +281,7 @@ inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld
  * projects u off the corrected basis (u - sum_{j<i} s_j V_j - (s_i - g.s) v_i).
  * alpha[i] = s_i - g.s - beta[i-1] g[i-1] (the only operator term that survives in a Krylov
  * basis, thick restarts included). Writes alpha[i], *beta_out = |u| and vnext = u / |u|
- * (when non-NULL). work: inch_dcgs2_work_size(J) doubles. J <= 256. */
+ * (when non-NULL). work: inch_dcgs2_work_size(J) doubles. */
 int64_t inch_dcgs2_work_size(int32_t J);
 inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
                             const double *beta, double *beta_out, double *vnext, double *work, void *stream);

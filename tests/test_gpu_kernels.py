@@ -324,7 +324,7 @@ def test_cgs2_reorth_fused(cuda, J, n):
 
 
 @pytest.mark.parametrize("i,kk,eps", [(0, 0, 1e-4), (1, 0, 1e-4), (20, 0, 1e-4), (20, 6, 1e-4), (20, 6, 0.0),
-                                      (40, 39, 1e-4)])
+                                      (40, 39, 1e-4), (300, 100, 1e-5)])
 def test_dcgs2_step_first_order_exact(cuda, i, kk, eps):
     """Delayed CGS2 on a provisional basis vector ~v_i = Q_i + sum_l eps_l Q_l (normalised), with
     kk restart (Ritz) rows in the projected matrix: the step must return the corrected
@@ -333,7 +333,7 @@ def test_dcgs2_step_first_order_exact(cuda, i, kk, eps):
     from inching_amd import ops
     print("case: dcgs2 i=%d kk=%d eps=%g" % (i, kk, eps), flush=True)
     rng = np.random.default_rng(100 * i + kk)
-    n = 300
+    n = max(300, i + 100)          # i = 300: the coefficients past 256 come from global memory
     Q, A, alpha, beta, theta, betak, T = projected_system(n, i, kk, rng)
     vt = Q[:, i] + Q[:, :i] @ (eps * rng.standard_normal(i))
     vt /= np.linalg.norm(vt)

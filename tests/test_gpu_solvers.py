@@ -50,6 +50,24 @@ def test_trlm_f64_hotelling(problem):
     _check(Ad, w, U, lam[idx], V[:, idx], k, 1e-9, 1e-10)
 
 
+def test_trlm_basis_over_256_vectors(problem):
+    """k = 90 with ncv = 270: the delayed-CGS2 step past 256 basis vectors (coefficients read
+    from global memory) against dense eigh."""
+    from inching_amd.operator import Operator
+    from inching_amd.trlm import TRLM
+    A, Ad, w, U, Q = problem
+    print("case: trlm f64 k=90 ncv=270", flush=True)
+    k = 90
+    op = Operator(A, "f64", Q=Q, shift=40.0)        # the 96th eigenvalue is 12.6: park the rigid modes at 41
+    t = TRLM(op, k, factor=3)
+    assert t.ncv == 270 and t.dcgs2
+    r = t.run(maxiter=500, tol=1e-10)
+    lam = r.eigval
+    V = r.eigvec.T.cpu().numpy()
+    idx = np.argsort(lam)
+    _check(Ad, w, U, lam[idx], V[:, idx], k, 1e-9, 1e-10)
+
+
 def test_mixed_precision_two_phase(problem):
     from inching_amd.pipeline import lowest_modes
     A, Ad, w, U, Q = problem
