@@ -315,8 +315,9 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     graph = None
     it = 0
     # lam_max for the relative residual: a 300-step Lanczos bound of the whole batch
-    from .chebyshev import spectrum_bound
-    lam_max = spectrum_bound(A, precision=precision)[1] if pol is None else float(pol.cc + pol.dd)
+    from .chebyshev import BOUND_STOP_REL, spectrum_bound
+    lam_max = (spectrum_bound(A, precision=precision, stop_rel=BOUND_STOP_REL)[1] if pol is None
+               else float(pol.cc + pol.dd))
     tol_abs = tol_rel * lam_max if pol is None else ptol * np.sqrt(k)
     conv = res <= tol_abs
     done_at[conv] = time.time() - t0
@@ -393,7 +394,7 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
     C = len(seg_atoms) - 1
     t_build = time.time() - t0
     ks = int(per_component) if per_component else int(np.ceil(1.15 * k_total / C)) + 4
-    lo, hi = CH.spectrum_bound(A, precision=precision) if filtered else (None, None)
+    lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL) if filtered else (None, None)
     for attempt in range(retries + 1):
         pol = None
         if filtered:

@@ -324,8 +324,20 @@ def optimize_on_memory(A, bound, wanted_left, wanted_count, designated_start=Non
     return pol, cnt, tuple(interval), kpm.n_spmv
 
 
-def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
-    """(min, max) Ritz values of a 300-step Lanczos run without re-orthogonalisation."""
+BOUND_STOP_REL = 1e-12   # early end of the bound run for callers that use only its maximum
+BOUND_CHECK_EVERY = 25
+
+
+def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64", stop_rel=None, info=None):
+    """(min, max) Ritz values of a 300-step Lanczos run without re-orthogonalisation
+    (``HermitianLanczos/T2.py:30-100``).
+
+    With ``stop_rel`` the run ends early, at a multiple of BOUND_CHECK_EVERY steps, once the
+    largest Ritz value moved by at most ``stop_rel`` (relative) over the last
+    BOUND_CHECK_EVERY steps. Only the maximum is meant to be used then: the largest Ritz
+    value converges first (on 300k-atom ANM globules it is within 1e-12 of the 300-step
+    value after 100-150 steps), while the minimum may still move. ``info["products"]``
+    receives the number of operator products."""
     if precision != "mf64":
         A.ensure(precision)
     n = A.shape[0]
@@ -355,16 +367,30 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64"):
     av(v, f)
     ops.multidot(row, 1, f, alpha[0:1], ws)
     ops.axpy_dev(alpha[0:1], -1.0, v, f)
+    m = n_tridiag + 1            # size of the tridiagonal matrix built
+    prev = None
     for i in range(n_tridiag):
         ops.gemv_update(row, 0, None, f, 0.0, nrm2_out=beta[i:i + 1], ws=ws)   # beta_i = ||f||
-        v0.copy_(v)
+        v0, v = v, v0            # the previous vector is kept by swapping buffers
         ops.scale_div(f, beta[i:i + 1], v)
+        row = v.view(1, -1)
         av(v, f)
         ops.axpy_dev(beta[i:i + 1], -1.0, v0, f)
         ops.multidot(row, 1, f, alpha[i + 1:i + 2], ws)
         ops.axpy_dev(alpha[i + 1:i + 2], -1.0, v, f)
-    a = alpha.cpu().numpy()
-    b = beta.cpu().numpy()
-    T = np.diag(a) + np.diag(b, -1) + np.diag(b, 1)
+        if stop_rel is not None and (i + 1) % BOUND_CHECK_EVERY == 0 and i + 1 < n_tridiag:
+            mm = i + 2
+            top = float(np.linalg.eigvalsh(_tridiag(alpha[:mm].cpu().numpy(), beta[:mm - 1].cpu().numpy()))[-1])
+            if prev is not None and abs(top - prev) <= stop_rel * abs(top):
+                m = mm
+                break
+            prev = top
+    if info is not None:
+        info["products"] = m
+    T = _tridiag(alpha[:m].cpu().numpy(), beta[:m - 1].cpu().numpy())
     w = np.linalg.eigvalsh(T)
     return float(np.min(w)), float(np.max(w))
+
+
+def _tridiag(a, b):
+    return np.diag(a) + np.diag(b, -1) + np.diag(b, 1)

@@ -91,9 +91,11 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
     torch.cuda.synchronize()
     t = time.time()
     if bound is None:
-        lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision)
+        binfo = {}
+        lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision,
+                                   stop_rel=CH.BOUND_STOP_REL, info=binfo)
         bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
-        n_spmv["bound"] = 301
+        n_spmv["bound"] = binfo["products"]
     lam_max = bound[1]
     phases["bound"] = time.time() - t
     pol = None
@@ -172,7 +174,7 @@ def lowest_modes_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
     the returned modes never depend on the network."""
     from .warmstart import warm_block
     t0 = time.time()
-    lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision)
+    lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision, stop_rel=CH.BOUND_STOP_REL)
     bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
     theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
     t_warm = time.time() - t0
@@ -204,9 +206,10 @@ def lowest_modes_jdm(A, k, Q=None, precision="mf64", tol_rel=1e-8, V0=None, shif
     t0 = time.time()
     n_spmv = {}
     if bound is None:
-        lo, hi = CH.spectrum_bound(A, precision=precision)
+        binfo = {}
+        lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL, info=binfo)
         bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
-        n_spmv["bound"] = 301
+        n_spmv["bound"] = binfo["products"]
     lam_max = bound[1]
     op = Operator(A, precision, Q=Q, shift=shift)
     t = time.time()
@@ -227,7 +230,7 @@ def lowest_modes_jdm_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **k
     tolerance, so the returned modes never depend on the network."""
     from .warmstart import warm_block
     t0 = time.time()
-    lo, hi = CH.spectrum_bound(A, precision=precision)
+    lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL)
     bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
     theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
     factor = int(kw.get("factor", 2))

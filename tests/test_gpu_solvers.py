@@ -50,6 +50,24 @@ def test_trlm_f64_hotelling(problem):
     _check(Ad, w, U, lam[idx], V[:, idx], k, 1e-9, 1e-10)
 
 
+def test_spectrum_bound_early_stop(problem):
+    """The bound run with an early stop ends before 300 steps with the same maximum: within
+    1e-10 of the 300-step value and of dense eigh's largest eigenvalue, never above it by
+    more than rounding; the products are counted."""
+    from inching_amd import chebyshev as CH
+    A, Ad, w, U, Q = problem
+    print("case: spectrum bound early stop", flush=True)
+    lo_f, hi_f = CH.spectrum_bound(A, precision="f64")
+    info = {}
+    lo_e, hi_e = CH.spectrum_bound(A, precision="f64", stop_rel=CH.BOUND_STOP_REL, info=info)
+    assert info["products"] < 301 and info["products"] % CH.BOUND_CHECK_EVERY == 1, info
+    assert abs(hi_e - hi_f) <= 1e-10 * hi_f
+    assert abs(hi_e - w[-1]) <= 1e-10 * w[-1] and hi_e <= w[-1] * (1 + 1e-14)
+    mf = {}
+    _, hi_m = CH.spectrum_bound(A, precision="mf64", stop_rel=CH.BOUND_STOP_REL, info=mf)
+    assert abs(hi_m - w[-1]) <= 1e-10 * w[-1], (hi_m, w[-1], mf)
+
+
 def test_trlm_basis_over_256_vectors(problem):
     """k = 90 with ncv = 270: the delayed-CGS2 step past 256 basis vectors (coefficients read
     from global memory) against dense eigh."""
