@@ -83,28 +83,45 @@ class Operator:
         self.n_spmv += 1
 
     def filtered(self, v64, u64):
-        """u = sum_k mu_k T_k((A - cc)/dd) v."""
+        """u = sum_k mu_k T_k((A - cc)/dd) v.
+
+        With a float64 operator the recurrence starts from v itself (it is only read) and,
+        when u does not share v's memory, accumulates straight into u; otherwise v is
+        converted into a scratch vector and the accumulator is converted into u at the end."""
         pol = self.pol
         dd, cc = float(pol.dd), float(pol.cc)
-        vk, vkm1, vkp1, acc = self._vk, self._vkm1, self._vkp1, self._acc
-        ops.convert(v64, vk)
+        scratch = [self._vk, self._vkm1, self._vkp1]
+        f64 = self.vdt == torch.float64
+        if f64 and v64.dtype == torch.float64 and v64.is_contiguous():
+            cur = v64
+        else:
+            cur = scratch[0]
+            ops.convert(v64, cur)
+        direct = (f64 and u64.dtype == torch.float64 and u64.is_contiguous()
+                  and u64.data_ptr() != v64.data_ptr())
+        acc = u64 if direct else self._acc
+        free = [b for b in scratch if b is not cur]
         ops.axpby(self.mu[0], v64, 0.0, acc)
         # the degree operator launches run back to back: one event pair times all of them
         if self.timer is not None:
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
+        prev = None
         for k in range(1, self.degree + 1):
+            out = free.pop(0)
             if k == 1:
-                self.spmv(vk, vkp1, _timed=False, s1=1.0 / dd, c0=cc, acc=acc, mu=self.mu[1])
+                self.spmv(cur, out, _timed=False, s1=1.0 / dd, c0=cc, acc=acc, mu=self.mu[1])
             else:
-                self.spmv(vk, vkp1, _timed=False, s1=2.0 / dd, c0=cc, z=vkm1, c2=1.0, acc=acc, mu=self.mu[k])
-            vkm1, vk, vkp1 = vk, vkp1, vkm1
+                self.spmv(cur, out, _timed=False, s1=2.0 / dd, c0=cc, z=prev, c2=1.0, acc=acc, mu=self.mu[k])
+            if prev is not None and prev is not v64:
+                free.append(prev)            # T_{k-2} v is no longer needed (v itself is never written)
+            prev, cur = cur, out
         if self.timer is not None:
             e1.record()
             self.timer.append((self.precision, "cheb", e0, e1, self.degree))
-        self._vk, self._vkm1, self._vkp1 = vk, vkm1, vkp1
-        ops.convert(acc, u64)
+        if not direct:
+            ops.convert(acc, u64)
 
     def apply(self, v64, u64):
         self.n_apply += 1
