@@ -5,7 +5,7 @@ and ``Tex/main.tex:63-146`` (Alg. 1):
 
 * degree[i] = |ball(i)| including i itself (:236-245);
 * seeds are taken in the order ``inds = argsort(degree)`` (:254); the reference uses
-  numpy's default (unstable) kind. This build pins the tie rule to a *stable*
+  numpy's default (unstable) kind. This restatement pins the tie rule to a *stable*
   argsort and also accepts an explicit seed order so any other tie rule can be
   reproduced (``seed_order=np.argsort(degree)`` reproduces the reference on a given
   numpy build);
