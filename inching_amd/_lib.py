@@ -23,6 +23,7 @@ SIGNATURES = {
     "inch_rcm_host": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _vp, _vp]),
     "inch_batch_head": (_STATUS, [_i64, _i64, _vp, _i64, _vp]),
     "inch_neighbors_celllist": (_STATUS, [_i64, _vp, _dbl, _vp, _vp, _i64, _vp, _vp]),
+    "inch_csr_permute": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "inch_hessian_assemble": (_STATUS, [_i64, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _vp, _vp]),
     "inch_bsr_spmv": (_STATUS, [_i64, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _dbl, _dbl, _vp, _dbl, _vp, _dbl,
                                 _vp, _vp, _i32, _i32, _vp]),

@@ -303,6 +303,58 @@ __global__ void __launch_bounds__(256) k_sort_rows(int64_t n, const int64_t *__r
   for (int t = threadIdx.x; t < len; t += blockDim.x) indices[b + t] = buf[t];
 }
 
+// counts[i] = length of row order[i] of the source CSR
+__global__ void k_perm_counts(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ order,
+                              int64_t *__restrict__ counts) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t o = order[i];
+  counts[i] = indptr[o + 1] - indptr[o];
+}
+
+// One block per new row: the columns of source row order[row], renumbered by undo (old index
+// -> new index), sorted ascending by a bitonic sort in LDS.
+__global__ void __launch_bounds__(256) k_permute_sort_rows(int64_t n, const int64_t *__restrict__ indptr,
+                                                            const int32_t *__restrict__ indices,
+                                                            const int32_t *__restrict__ order,
+                                                            const int64_t *__restrict__ undo,
+                                                            const int64_t *__restrict__ out_ptr,
+                                                            int32_t *__restrict__ out_idx,
+                                                            int32_t *__restrict__ overflow) {
+  __shared__ int32_t buf[kMaxRow];
+  const int64_t row = blockIdx.x;
+  if (row >= n) return;
+  const int64_t o = order[row];
+  const int64_t b = indptr[o];
+  const int len = (int)(indptr[o + 1] - b);
+  if (len > kMaxRow) {
+    if (threadIdx.x == 0) atomicMax(overflow, len);
+    return;
+  }
+  int p2 = 1;
+  while (p2 < len) p2 <<= 1;
+  for (int t = threadIdx.x; t < p2; t += blockDim.x) buf[t] = t < len ? (int32_t)undo[indices[b + t]] : 0x7fffffff;
+  __syncthreads();
+  for (int k = 2; k <= p2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < p2; t += blockDim.x) {
+        const int ixj = t ^ j;
+        if (ixj > t) {
+          const bool up = (t & k) == 0;
+          const int32_t x = buf[t], y = buf[ixj];
+          if ((x > y) == up) {
+            buf[t] = y;
+            buf[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int64_t ob = out_ptr[row];
+  for (int t = threadIdx.x; t < len; t += blockDim.x) out_idx[ob + t] = buf[t];
+}
+
 // One 64-lane wave per row: H_ij = -d d^T / r^2 for j != i (r^2 > 0); the diagonal
 // block is minus the sum of the row's off-diagonal blocks plus plus_i * I.
 __global__ void __launch_bounds__(256) k_assemble(int64_t n, const double *__restrict__ xyz, double plus_i,
@@ -658,6 +710,30 @@ extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, dou
   HIP_TRY(hipStreamSynchronize(st));
   if (over) INCH_FAIL(INCH_ERR_CAPACITY, "inch_neighbors_celllist: a row has %d neighbours (> %d)", over, kMaxRow);
   if (nnz_out) *nnz_out = total;
+  return INCH_OK;
+}
+
+extern "C" inch_status inch_csr_permute(int64_t n, const int64_t *indptr, const int32_t *indices,
+                                        const int32_t *order, const int64_t *undo, int64_t *out_indptr,
+                                        int32_t *out_indices, void *stream) {
+  INCH_REQUIRE(n > 0 && indptr && indices && order && undo && out_indptr && out_indices, "inch_csr_permute: bad args");
+  hipStream_t st = (hipStream_t)stream;
+  DevBuf counts, ovf;
+  HIP_TRY(hipMalloc(&counts.p, n * sizeof(int64_t)));
+  HIP_TRY(hipMalloc(&ovf.p, sizeof(int32_t)));
+  HIP_TRY(hipMemsetAsync(ovf.p, 0, sizeof(int32_t), st));
+  const int bs = 256;
+  k_perm_counts<<<(unsigned)((n + bs - 1) / bs), bs, 0, st>>>(n, indptr, order, (int64_t *)counts.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_i64<<<1, 1024, 0, st>>>(n, (int64_t *)counts.p, out_indptr);
+  HIP_TRY(hipGetLastError());
+  k_permute_sort_rows<<<(unsigned)n, 256, 0, st>>>(n, indptr, indices, order, undo, out_indptr, out_indices,
+                                                  (int32_t *)ovf.p);
+  HIP_TRY(hipGetLastError());
+  int32_t over = 0;
+  HIP_TRY(hipMemcpyAsync(&over, ovf.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (over) INCH_FAIL(INCH_ERR_CAPACITY, "inch_csr_permute: a row has %d entries (> %d)", over, kMaxRow);
   return INCH_OK;
 }
 

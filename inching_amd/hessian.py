@@ -242,12 +242,15 @@ def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_orde
         return ops.neighbours_pbc(xyz_, rc, box, origin)
 
     comps = None                     # connected components (offsets in the RCM order)
+    permuted = None                  # the search's CSR renumbered into the RCM order
     if reorder:
         indptr, indices, _ = pattern(xyz)
         if rcm == "device":
             order_d, undo_d, comps = ops.rcm_device(indptr, indices, seed_order=seed_order, components=True)
             xyz = xyz[order_d.long()].contiguous()
             order, undo = order_d.cpu().numpy(), undo_d.cpu().numpy()
+            if box is None:
+                permuted = ops.csr_permute(indptr, indices, order_d, undo_d)
         else:
             order, undo = ops.rcm_host(indptr.cpu().numpy(), indices.cpu().numpy(), seed_order=seed_order)
             xyz = xyz[torch.as_tensor(order.astype(np.int64), device=device)].contiguous()
@@ -255,7 +258,10 @@ def build_hessian(X, rc, plus_i=1.0, reorder=True, precision=("f32",), seed_orde
     else:
         order = np.arange(n, dtype=np.int32)
         undo = np.arange(n, dtype=np.int64)
-    rowptr, col, images = pattern(xyz)
+    if permuted is not None:
+        (rowptr, col), images = permuted, None
+    else:
+        rowptr, col, images = pattern(xyz)
     if images is None:
         v64, v32, diag = ops.hessian_assemble(xyz, rc, plus_i, rowptr, col, want64="f64" in precision,
                                               want32="f32" in precision)

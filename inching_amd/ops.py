@@ -133,6 +133,21 @@ def neighbours(xyz, rc, stream=None):
     return indptr, indices[:nnz.value]
 
 
+def csr_permute(indptr, indices, order, undo, stream=None):
+    """The neighbour CSR renumbered by a permutation (inch_csr_permute): order int32 [n]
+    (new -> old), undo int64 [n] (old -> new), device tensors. Returns (indptr, indices) of
+    the reordered atoms, equal to a fresh neighbour search of the reordered coordinates."""
+    require_gpu()
+    n = indptr.numel() - 1
+    if (order.dtype != torch.int32 or undo.dtype != torch.int64 or order.numel() != n or undo.numel() != n
+            or indptr.dtype != torch.int64 or indices.dtype != torch.int32):
+        raise ValueError("csr_permute: indptr int64 [n+1], indices int32, order int32 [n], undo int64 [n]")
+    out_ptr = torch.empty(n + 1, dtype=torch.int64, device=indptr.device)
+    out_idx = torch.empty(max(1, indices.numel()), dtype=torch.int32, device=indptr.device)
+    call("inch_csr_permute", n, P(indptr), P(indices), P(order), P(undo), P(out_ptr), P(out_idx), stream_ptr(stream))
+    return out_ptr, out_idx[:indices.numel()]
+
+
 def neighbours_pbc(xyz, rc, box, origin, stream=None):
     """Neighbour CSR with periodic images in x and y: (indptr, indices, images), images[e]
     the 9-bit mask of the images of column j within the cutoff (see inch_neighbors_celllist_pbc).

@@ -109,6 +109,14 @@ inch_status inch_batch_head(int64_t n, int64_t maxleaf, int64_t *head, int64_t c
 inch_status inch_neighbors_celllist(int64_t n, const double *xyz, double rc, int64_t *indptr,
                                     int32_t *indices, int64_t capacity, int64_t *nnz_out,
                                     void *stream);
+/* The neighbour CSR renumbered by a permutation (no reference counterpart as one call: the
+ * reference searches the reordered coordinates again, Fuel/Coordinate/T1.py:331-481 after
+ * :171-323). order[i] = old index of new atom i (int32), undo[old] = new index (int64);
+ * row i of the output is row order[i] of the input with every column j replaced by undo[j],
+ * ascending: the CSR a fresh search of the reordered coordinates returns. out_indptr [n+1],
+ * out_indices [nnz]. Rows over 2048 entries return INCH_ERR_CAPACITY. Syncs the stream. */
+inch_status inch_csr_permute(int64_t n, const int64_t *indptr, const int32_t *indices, const int32_t *order,
+                             const int64_t *undo, int64_t *out_indptr, int32_t *out_indices, void *stream);
 
 /* ANM Hessian A = H + plus_i*I into full-storage block rows on the pattern
  * (rowptr, col) (the neighbour CSR). vals64 (float64[nnzb*9]) and/or vals32

@@ -352,6 +352,32 @@ def test_dcgs2_step_first_order_exact(cuda, i, kk, eps):
     assert np.linalg.norm(Vo[i + 1] - Q[:, i + 1]) < bar
 
 
+@pytest.mark.parametrize("kind", ["rcm", "random"])
+def test_csr_permute_equals_fresh_search(glob2k, kind):
+    """The search's CSR renumbered by a permutation equals, bit for bit, a fresh neighbour
+    search of the permuted coordinates (RCM order and a random order); build_hessian's
+    pattern in RCM order equals a fresh search too."""
+    from inching_amd import ops
+    from inching_amd.hessian import build_hessian
+    print("case: csr permute %s" % kind, flush=True)
+    xyz = _dev(glob2k)
+    ip, ix = ops.neighbours(xyz, 8.0)
+    n = xyz.shape[0]
+    if kind == "rcm":
+        order, undo = ops.rcm_device(ip, ix)
+    else:
+        perm = np.random.default_rng(5).permutation(n)
+        order = _dev(perm.astype(np.int32))
+        undo = _dev(np.argsort(perm).astype(np.int64))
+    pp, px = ops.csr_permute(ip, ix, order, undo)
+    fp, fx = ops.neighbours(xyz[order.long()].contiguous(), 8.0)
+    assert torch.equal(pp, fp) and torch.equal(px, fx)
+    if kind == "rcm":
+        A = build_hessian(glob2k, 8.0, plus_i=1.0, reorder=True, precision=())
+        gp, gx = ops.neighbours(A.xyz, 8.0)
+        assert torch.equal(A.rowptr, gp) and torch.equal(A.col, gx)
+
+
 def test_spmv_col16_bitwise_equal(mat):
     """16-bit column offsets (row base + offset) give bitwise the int32-column result for every
     lane count and dtype pair; a row spanning more than 16 bits is rejected."""
