@@ -25,14 +25,15 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
   return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-// part[c*J + j] = sum_{i in chunk c} V[j*ld + i] * u[i]
+// part[c*J + j] = sum_{i in chunk c} V[j*ld + i] * u[i]; blockIdx.y picks the tile of 8 rows
 __global__ void __launch_bounds__(256) k_seg_dot_part(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
                                                        int J, const double *__restrict__ V, int64_t ld,
                                                        const double *__restrict__ u, double *__restrict__ part) {
   __shared__ double red[4];
   const int c = blockIdx.x;
   const int64_t b = cbeg[c], e = cend[c];
-  for (int j0 = 0; j0 < J; j0 += 8) {
+  {
+    const int j0 = 8 * (int)blockIdx.y;
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
       const double ui = u[i];
@@ -147,7 +148,8 @@ __global__ void __launch_bounds__(256) k_seg_cgs_b(const int64_t *__restrict__ c
 
 // Delayed CGS2 step, per segment (the segmented form of inch_dcgs2_step): i = J - 1, V[i] is
 // the provisional vector, u = Op V[i].
-// part[c*2J + j] = V_j . u and part[c*2J + J + j] = V_j . w over chunk c (w = V[J-1]).
+// part[c*2J + j] = V_j . u and part[c*2J + J + j] = V_j . w over chunk c (w = V[J-1]);
+// blockIdx.y picks the tile of 8 rows.
 __global__ void __launch_bounds__(256) k_seg_dual_dot_part(const int64_t *__restrict__ cbeg,
                                                             const int64_t *__restrict__ cend, int J,
                                                             const double *__restrict__ V, int64_t ld,
@@ -156,7 +158,8 @@ __global__ void __launch_bounds__(256) k_seg_dual_dot_part(const int64_t *__rest
   __shared__ double red[4];
   const int c = blockIdx.x;
   const int64_t b = cbeg[c], e = cend[c];
-  for (int j0 = 0; j0 < J; j0 += 8) {
+  {
+    const int j0 = 8 * (int)blockIdx.y;
     double a[8] = {0, 0, 0, 0, 0, 0, 0, 0}, g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
       const double ui = u[i], wi = w[i];
@@ -319,7 +322,7 @@ inch_status inch_seg_multidot(int32_t nchunk, int32_t S, const int64_t *cbeg, co
   INCH_REQUIRE(nchunk > 0 && S > 0 && J > 0 && cbeg && cend && sptr && V && u && h && work,
                "inch_seg_multidot: bad args");
   hipStream_t st = (hipStream_t)stream;
-  k_seg_dot_part<<<nchunk, 256, 0, st>>>(cbeg, cend, J, V, ld, u, work);
+  k_seg_dot_part<<<dim3(nchunk, (J + 7) / 8), 256, 0, st>>>(cbeg, cend, J, V, ld, u, work);
   HIP_TRY(hipGetLastError());
   const int64_t tot = (int64_t)S * J;
   k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, J, sptr, work, h, 0);
@@ -383,7 +386,8 @@ inch_status inch_seg_dcgs2_step(int32_t nchunk, int32_t S, const int64_t *cbeg, 
   double *sg = part + (int64_t)nchunk * 2 * J;
   double *coef = sg + (int64_t)S * 2 * J;
   double *npart = coef + (int64_t)S * 2 * J;
-  k_seg_dual_dot_part<<<nchunk, 256, 0, st>>>(cbeg, cend, J, V, ld, u, V + (int64_t)(J - 1) * ld, part);
+  k_seg_dual_dot_part<<<dim3(nchunk, (J + 7) / 8), 256, 0, st>>>(cbeg, cend, J, V, ld, u, V + (int64_t)(J - 1) * ld,
+                                                                 part);
   HIP_TRY(hipGetLastError());
   const int64_t tot = (int64_t)S * 2 * J;
   k_seg_reduce<<<(unsigned)((tot + 255) / 256), 256, 0, st>>>(S, 2 * J, sptr, part, sg, 0);
