@@ -50,6 +50,32 @@ def test_trlm_f64_hotelling(problem):
     _check(Ad, w, U, lam[idx], V[:, idx], k, 1e-9, 1e-10)
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-10), ("mf64", 1e-10), ("f32", 2e-5)])
+def test_operator_filter_equals_spectral_filter(problem, precision, tol):
+    """Operator.filtered (the Chebyshev recurrence as fused operator launches) equals
+    U p(w) U^T v from dense eigh, p the designed filter polynomial; the in-place call
+    (output = input) gives bitwise the out-of-place result."""
+    import torch
+    from inching_amd import chebyshev as CH
+    from inching_amd.operator import Operator
+    A, Ad, w, U, Q = problem
+    print("case: operator filter %s" % precision, flush=True)
+    pol = CH.design_filter(1.0, w[40], (1.0, w[-1] * (1 + 1e-3)), 5120, 5, 0.7)
+    m = int(pol.AdjustedDegree)
+    p = np.polynomial.chebyshev.chebval((w - pol.cc) / pol.dd, np.asarray(pol.AdjustedCoefficients)[:m + 1])
+    v = np.random.default_rng(3).standard_normal(Ad.shape[0])
+    ref = U @ (p * (U.T @ v))
+    op = Operator(A, precision, pol=pol)
+    vd = torch.as_tensor(v, device="cuda")
+    u = torch.empty_like(vd)
+    op.apply(vd, u)
+    got = u.cpu().numpy()
+    assert np.linalg.norm(got - ref) <= tol * np.linalg.norm(ref), np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    x = vd.clone()
+    op.apply(x, x)
+    assert torch.equal(x, u)
+
+
 def test_spectrum_bound_early_stop(problem):
     """The bound run with an early stop ends before 300 steps with the same maximum: within
     1e-10 of the 300-step value and of dense eigh's largest eigenvalue, never above it by
