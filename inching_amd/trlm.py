@@ -1,4 +1,4 @@
-"""Thick-restart Lanczos with full CGS2 re-orthogonalisation on the device.
+"""Thick-restart Lanczos with full (delayed) CGS2 re-orthogonalisation on the device.
 
 Follows ``Burn/ThickRestartLanczosHotellingDeflation/T1.py``:
 ``S_HeigvalTRLMHD_HeigvecTRLMHD`` :60-310 (driver and thick restart),
@@ -8,10 +8,13 @@ smallest k unfiltered / largest k filtered), ``OOC7_A_RitzV_u_alpha_beta_istart_
 CGS pass 2, v_next = u / beta).
 
 Device side: the basis V [ncv, n] float64 is row-major (one Lanczos vector per row); the
-projections are the multi-dot and rank-J update kernels (deterministic reductions); the
-restart combination V[:k] = W^T V is ``inch_rr_combine``. alpha = V[i].u is read off the
-first projection (h[i]) instead of a separate dot. Only the (ncv x ncv) projected
-eigenproblem goes to the host (numpy eigh), once per restart, as in the reference.
+restart combination V[:k] = W^T V is ``inch_rr_combine``. Each step orthogonalises with
+delayed CGS2 (``inch_dcgs2_step``, the default): the reference's second CGS pass of step
+i - 1 is done inside step i, in the same read of V as step i's first pass, so a step reads V
+twice instead of three times; the last residual of a cycle gets its second pass before the
+restart. ``dcgs2=False`` selects the fused CGS2 kernels instead (the reference's order of
+passes). Reductions are deterministic. Only the (ncv x ncv) projected eigenproblem goes to
+the host (numpy eigh), once per restart, as in the reference.
 """
 import time
 
