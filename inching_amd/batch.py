@@ -7,7 +7,8 @@ far apart in one frame, so a single cell-list search, RCM ordering (the BFS keep
 connected structure contiguous) and assembly build all of them. Thick-restart Lanczos
 then runs every structure in lockstep: the operator is one launch over the whole batch,
 and every inner product, norm, update and restart combination is a *segmented* kernel
-over the structures' dof ranges (``inch_seg_*``). The restart body of all structures is
+over the structures' dof ranges (``inch_seg_*``; each Lanczos step orthogonalises with the
+segmented delayed CGS2, two reads of the basis). The restart body of all structures is
 captured once into a HIP graph and replayed. The projected eigenproblems (one arrowhead
 matrix per structure) are solved on the host as one batched ``numpy.linalg.eigh``.
 
