@@ -91,12 +91,14 @@ def ungapped_islands(indptr, indices, n_atoms, maxleafsize=1000, gap_size=100):
 
 def X_KdUngappedMinMaxNeighbor(X, rc_Gamma=15.0, maxleafsize=1000, CollectStat=False, User_ReturnHalfNnz=False,
                                User_GapSize=100, User_DictCharmmGuiPbc=None, SliceForm=True):
-    """{batch: [(start, end), ...]} column islands (slice form) [, HalfNnz]."""
+    """{batch: [(start, end), ...]} column islands [, HalfNnz].
+
+    The islands are in slice form (end exclusive) whatever ``SliceForm`` says: the reference
+    returns its slice-form dictionary in both cases (``:470-481``); ``SliceForm`` only shaped a
+    per-batch (min, max) dictionary that it never returned."""
     ip, ix = _neighbours(X, rc_Gamma / 10.0, User_DictCharmmGuiPbc)
     n = ip.shape[0] - 1
     islands, half = ungapped_islands(ip, ix, n, maxleafsize=maxleafsize, gap_size=User_GapSize)
-    if not SliceForm:
-        islands = {b: [(s, e - 1) for (s, e) in v] for b, v in islands.items()}
     if User_ReturnHalfNnz:
         return islands, half
     return islands

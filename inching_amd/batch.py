@@ -322,6 +322,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     tol_abs = tol_rel * lam_max if pol is None else ptol * np.sqrt(k)
     conv = res <= tol_abs
     done_at[conv] = time.time() - t0
+    cycles = 0                                   # restart cycles that ran (body() launches)
     for it in range(maxiter):
         if conv.all():
             break
@@ -334,6 +335,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
             graph.replay()
         else:
             body()
+        cycles += 1
         a, b = alpha.cpu().numpy().copy(), beta.cpu().numpy().copy()
         a[:k] = Sel.T
         b[:k] = 0.0
@@ -368,9 +370,9 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
         rel = np.take_along_axis(rel, ordq, axis=1)
     order = np.argsort(seg_struct)
     done = np.where(np.isnan(done_at), total, done_at)
-    n_steps = ncv + (it + 1) * (ncv - k)        # operator applications of the lockstep Lanczos
+    n_steps = ncv + cycles * (ncv - k)          # operator applications of the lockstep Lanczos
     return BatchResult(eigval=Sel[order], rel_res=rel[order], latency=done[order], seconds=total,
-                       restarts=it + 1, n_spmv=n_steps, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
+                       restarts=cycles, n_spmv=n_steps, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
                        seg_struct=seg_struct, A=A, eigvec=Xr, converged=int(conv.sum()))
 
 

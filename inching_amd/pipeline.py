@@ -18,6 +18,7 @@ Acceptance is always the float64 residual ``||A v - lam v|| / (lam_max ||v||)`` 
 with the matrix-free float64 operator, after a float64 Rayleigh-Ritz polish.
 """
 import time
+import warnings
 
 import numpy as np
 import torch
@@ -37,6 +38,29 @@ class ModeResult:
 
 
 STOP_SAFETY = 0.3   # a cycle ends when the predicted worst relative residual is 0.3 x the target
+
+
+class NotConvergedWarning(RuntimeWarning):
+    """The solver ran out of iterations before every pair met the tolerance; the result holds
+    the best pairs found (the reference prints an 'ABORTED. It did not actually converged!'
+    line and returns its partial result, ``JacobiDavidsonHotellingDeflation/T1.py:745-757``)."""
+
+
+def _final(A, Q, lam_max, tol_rel, k, X, state, solver, iters):
+    """(theta, X, rel, converged) of the last float64 check of the Ritz rows X [k, n]; when no check ran (the Lanczos
+    estimate never came within reach of the target before maxiter), one float64 polish of
+    the current Ritz block. Warns when the worst relative residual misses ``tol_rel``."""
+    if state["last"] is None:
+        theta, Xp, info = polish(A, X[:k], Q=Q, lam_max=lam_max)
+        state["last"] = (theta, Xp, info["res"] / lam_max)
+    theta, Xp, rel = state["last"]
+    converged = bool(np.max(rel) <= tol_rel)
+    if not converged:
+        warnings.warn("ABORTED. It did not actually converge: %s stopped after %d restarts with a worst relative "
+                      "residual of %.3e > %.1e; returning the best pairs found" % (solver, iters, float(np.max(rel)),
+                                                                                   tol_rel),
+                      NotConvergedWarning, stacklevel=3)
+    return theta, Xp, rel, converged
 
 
 def _check_factory(A, Q, lam_max, tol_rel, k, stall=None):
@@ -132,6 +156,9 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
         n_spmv["phase1_f32"] = op32.n_spmv
         total += op32.n_spmv
         applies += op32.n_apply
+        if state1["last"] is None:
+            theta1, X1, info1 = polish(A, r1.eigvec, Q=Q, lam_max=lam_max)
+            state1["last"] = (theta1, X1, info1["res"] / lam_max)
         theta1, X1, rel1 = state1["last"]
         if float(np.max(rel1)) <= tol_rel:
             torch.cuda.synchronize()
@@ -139,7 +166,7 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
                               phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r1.iters + 1, n_apply=applies,
                               ncv=min(max(factor * k, k + 32), A.shape[0] - 1),
                               degree=int(pol.AdjustedDegree) if pol is not None else 0,
-                              seconds=time.time() - t_all, precision=precision)
+                              seconds=time.time() - t_all, precision=precision, converged=True)
         g = torch.Generator(device=A.device)
         g.manual_seed(seed + 1)
         c = torch.rand(k, generator=g, dtype=torch.float64, device=A.device) + 0.5
@@ -158,13 +185,13 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
     n_spmv[key] = op64.n_spmv
     total += op64.n_spmv
     applies += op64.n_apply
-    theta, X, rel = state["last"]
+    theta, X, rel, converged = _final(A, Q, lam_max, tol_rel, k, r2.eigvec, state, "TRLM", r2.iters + 1)
     torch.cuda.synchronize()
     return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(rel), lam_max=lam_max,
                       phases=phases, n_spmv=n_spmv, total_spmv=total, restarts=r2.iters + 1, n_apply=applies,
                       ncv=min(max(factor * k, k + 32), A.shape[0] - 1),
                       degree=int(pol.AdjustedDegree) if pol is not None else 0,
-                      seconds=time.time() - t_all, precision=precision)
+                      seconds=time.time() - t_all, precision=precision, converged=converged)
 
 
 def lowest_modes_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
@@ -178,22 +205,36 @@ def lowest_modes_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
     bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
     theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
     t_warm = time.time() - t0
+    if not _usable_block(theta, info):
+        return _cold_fallback(lowest_modes, A, k, Q, precision, tol_rel, bound, t0, kw)
     m = min(k, X.shape[0])
     g = torch.Generator(device=A.device)
     g.manual_seed(int(kw.get("seed", 0)) + 7)
     c = torch.rand(m, generator=g, dtype=torch.float64, device=A.device) + 0.5
     v0 = (c[:, None] * X[:m]).sum(0)
-    r = lowest_modes(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, v0=v0, **kw)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", NotConvergedWarning)     # the cold start below decides
+        r = lowest_modes(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, v0=v0, **kw)
     r.phases["warm_block"] = t_warm
     r.seconds = time.time() - t0
     r.warm_rel_res = info.get("rel_res")
     r.fallback = False
-    if float(np.max(r.rel_res)) > tol_rel:
-        cold = lowest_modes(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, **kw)
-        cold.fallback = True
-        cold.seconds = time.time() - t0
-        return cold
+    if not r.converged:
+        return _cold_fallback(lowest_modes, A, k, Q, precision, tol_rel, bound, t0, kw)
     return r
+
+
+def _usable_block(theta, info):
+    """A warm block is used only when its Rayleigh-Ritz values and residuals are finite (a
+    network with corrupted weights predicts NaN or Inf)."""
+    return bool(np.all(np.isfinite(theta)) and np.all(np.isfinite(info["res"])))
+
+
+def _cold_fallback(solver, A, k, Q, precision, tol_rel, bound, t0, kw):
+    cold = solver(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, **kw)
+    cold.fallback = True
+    cold.seconds = time.time() - t0
+    return cold
 
 
 def lowest_modes_jdm(A, k, Q=None, precision="mf64", tol_rel=1e-8, V0=None, shift=40.0, factor=2, maxiter=5000,
@@ -217,11 +258,13 @@ def lowest_modes_jdm(A, k, Q=None, precision="mf64", tol_rel=1e-8, V0=None, shif
     phases = {"solve_jdm_" + precision: time.time() - t}
     n_spmv["jdm"] = op.n_spmv
     theta, X, info = polish(A, r.eigvec, Q=Q, lam_max=lam_max)
+    state = {"last": (theta, X, info["rel_res"])}
+    theta, X, rel, converged = _final(A, Q, lam_max, tol_rel, k, X, state, "JDM", int(r.iters))
     torch.cuda.synchronize()
-    return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(info["rel_res"]), lam_max=lam_max,
+    return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(rel), lam_max=lam_max,
                       phases=phases, n_spmv=n_spmv, total_spmv=op.n_spmv, restarts=int(r.iters), n_apply=op.n_apply,
                       ncv=factor * k, degree=0, seconds=time.time() - t0, precision=precision,
-                      converged=int(r.converged))
+                      n_converged=int(r.converged), converged=converged and int(r.converged) >= k)
 
 
 def lowest_modes_jdm_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **kw):
@@ -233,16 +276,17 @@ def lowest_modes_jdm_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **k
     lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL)
     bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
     theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
+    if not _usable_block(theta, info):
+        return _cold_fallback(lowest_modes_jdm, A, k, Q, precision, tol_rel, bound, t0, kw)
     factor = int(kw.get("factor", 2))
     V0 = X[:min(X.shape[0], factor * k - 1)].contiguous()
-    r = lowest_modes_jdm(A, k, Q=Q, precision=precision, tol_rel=tol_rel, V0=V0, bound=bound, **kw)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", NotConvergedWarning)     # the cold start below decides
+        r = lowest_modes_jdm(A, k, Q=Q, precision=precision, tol_rel=tol_rel, V0=V0, bound=bound, **kw)
     r.phases["warm_block"] = time.time() - t0 - r.seconds
     r.seconds = time.time() - t0
     r.warm_rel_res = info.get("rel_res")
     r.fallback = False
-    if r.converged < k or float(np.max(r.rel_res)) > tol_rel:
-        cold = lowest_modes_jdm(A, k, Q=Q, precision=precision, tol_rel=tol_rel, bound=bound, **kw)
-        cold.fallback = True
-        cold.seconds = time.time() - t0
-        return cold
+    if not r.converged:
+        return _cold_fallback(lowest_modes_jdm, A, k, Q, precision, tol_rel, bound, t0, kw)
     return r

@@ -195,10 +195,15 @@ def irlm(apply_A, n, k, rng, maxiter=15000, tol=1e-8, Q=None, shift=10.0):
         V[:k] = Qt[:k] @ V[:k + p]
         V[k] = b[k - 1] * v + (b[k + p - 1] * Qt[k - 1, k + p - 1]) * V[k + p]
         bb = np.sqrt(np.sum(V[k] * V[k]))
-        V[k] /= bb
         b[k - 1] = bb
-        _orth_all_before(V, k, k)
-        _orth_all_before(V, k, k)
+        # A zero residual (an invariant subspace) leaves V[k] = 0 / 0 in the reference
+        # (IRLM/T1.py:289-290); its next Lanczos call replaces V[k] by a random vector
+        # because b[k - 1] < tol (:118-125), so skipping the normalisation here gives the
+        # same iteration without the division by zero.
+        if bb > 0.0:
+            V[k] /= bb
+            _orth_all_before(V, k, k)
+            _orth_all_before(V, k, k)
         return a, b
 
     lanczos(1, k)

@@ -146,3 +146,22 @@ def test_trlm_filtered(problem):
     rq = rayleigh_quotients(A, r.eigvec).cpu().numpy()
     idx = np.argsort(rq)
     _check(Ad, w, U, rq[idx], r.eigvec.T.cpu().numpy()[:, idx], k, 1e-9, 1e-9)
+
+
+@pytest.mark.parametrize("designated", [None, "tenth"])
+def test_device_kpm_bisection_interval_holds_wanted_count(problem, designated):
+    """The device bisection (PolynomialFilters/T2.py:142-257 with KPM moments from the device
+    operator) settles on an interval whose exact eigenvalue count (dense eigh) is the wanted
+    count within the margin plus the spread of 64 KPM probes."""
+    from inching_amd import chebyshev as CH
+    A, Ad, w, U, Q = problem
+    print("case: device KPM bisection %s" % designated, flush=True)
+    bound = (1.0, w[-1] * (1 + 1e-3))
+    wanted, margin = 80, 8
+    start = bound[1] / 10.0 if designated == "tenth" else None
+    pol, cnt, interval, nk = CH.optimize_on_memory(A, bound, 1.0, wanted_count=wanted, designated_start=start,
+                                                   max_degree=5120, min_degree=5, margin=margin, trials=64,
+                                                   ratio=0.7, seed=3, precision="f64")
+    assert abs(cnt - wanted) <= margin and nk > 0
+    exact = int(np.sum((w >= 1.0 - 1e-9) & (w <= interval[1])))
+    assert abs(exact - wanted) <= margin + 8, (exact, interval, cnt)

@@ -224,6 +224,61 @@ def test_islands_cover_all_columns():
         assert isl[b][0][0] == cols[0] and isl[b][-1][1] == cols[-1] + 1
 
 
+# Batch 4 of n = 1000 atoms with leaves <= 100 is rows [255, 318) (the halving gives leaf
+# sizes 64 x3, 63 x5, 62 x5, 61 x3, sorted descending). Its union of columns has 87
+# values; the differences of interest (index iii: left value -> right value):
+#   iii = 1  : 11 -> 131 (120 > 100)  index < 5            -> never a gap
+#   iii = 6  : 135 -> 252 (117)       island began at 10, 10 + 5 < 255 -> gap [136, 252)
+#   iii = 75 : 320 -> 440 (120)       island began at 252, 252 + 5 >= 255 and 321 <= 255 + 63 + 5
+#                                      -> the diagonal exception, no gap
+#   iii = 78 : 442 -> 580 (138)       443 > 323 -> gap [443, 580)
+#   iii = 79 : 580 -> 680 (100)       not > 100 -> no gap
+#   iii = 82 : 682 -> 800 (118)       iii == 87 - 5 -> never a gap
+# Walking Fuel/Coordinate/T1.py:401-446 by hand gives islands (10,136), (252,443), (580,804).
+_KNOWN_UNION = ([10, 11] + list(range(131, 136)) + list(range(252, 321)) + [440, 441, 442, 580, 680, 681, 682]
+                + [800, 801, 802, 803])
+_KNOWN_ISLANDS = [(10, 136), (252, 443), (580, 804)]
+
+
+def _known_island_csr():
+    """Every row lists only itself, except row 255, which lists the whole designed union."""
+    n = 1000
+    rows = [[r] for r in range(n)]
+    rows[255] = sorted(_KNOWN_UNION)
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    indptr[1:] = np.cumsum([len(r) for r in rows])
+    return indptr, np.concatenate(rows).astype(np.int64)
+
+
+def test_islands_known_answer_hand_derived():
+    """Each exception of the reference's gap rule (index < 5, index len - 5, the diagonal
+    block, strictly wider than GapSize) is planted once; dropping or loosening any one of
+    them changes the batch-4 islands."""
+    assert len(_KNOWN_UNION) == 87 and len(set(_KNOWN_UNION)) == 87
+    ip, ix = _known_island_csr()
+    head = OP.batch_head(1000, 100)
+    assert head.tolist() == [0, 64, 128, 192, 255, 318, 381, 444, 507, 569, 631, 693, 755, 817, 878, 939, 1000]
+    isl, half = OP.ungapped_islands(ip, ix, maxleafsize=100, gap_size=100)
+    assert isl[4] == _KNOWN_ISLANDS
+    for b in range(16):
+        if b != 4:
+            assert isl[b] == [(int(head[b]), int(head[b + 1]))]
+    # HalfNnz = ((sum of row lengths - n) / 2 + n) * 9 with sum = 999 + 87
+    assert half == 9387.0
+    # the same CSR with a gap rule of 99: iii = 79 (580 -> 680, 100 > 99) is now cut as well
+    isl99, _ = OP.ungapped_islands(ip, ix, maxleafsize=100, gap_size=99)
+    assert isl99[4] == [(10, 136), (252, 443), (580, 581), (680, 804)]
+
+
+def test_islands_shim_known_answer():
+    """The drop-in's island builder (host code over the library's batch partition) on the
+    same hand-built CSR, and slice form whatever SliceForm says, as the reference returns."""
+    from InchingLiteInteger.Fuel.Coordinate.T1 import ungapped_islands
+    ip, ix = _known_island_csr()
+    isl, half = ungapped_islands(ip, ix, 1000, maxleafsize=100, gap_size=100)
+    assert isl[4] == _KNOWN_ISLANDS and half == 9387.0
+
+
 # ------------------------------------------------------------------- Hessian --
 def test_two_atom_hessian_closed_form():
     X = np.array([[0.0, 0.0, 0.0], [1.0, 2.0, 2.0]])
@@ -362,6 +417,45 @@ def test_kpm_count_is_close_to_exact():
     est = OC.kpm_estimate(lambda x: A @ x, A.shape[0], pol, 40, np.random.default_rng(0))
     exact = int(np.sum((w >= 1.0 - 1e-9) & (w <= 2.0)))
     assert abs(est - exact) <= max(10, 0.35 * exact)
+
+
+@pytest.mark.parametrize("designated", [None, 4.0, 1.5])
+def test_optimize_on_memory_interval_holds_wanted_count(designated, monkeypatch):
+    """The bisection of PolynomialFilters/T2.py:142-257 on a matrix whose spectrum is known
+    in closed form (diagonal, eigenvalues 1 + 59 t^2 on a uniform grid of t): the interval it
+    settles on must hold the wanted number of eigenvalues to within the margin plus the KPM
+    noise of 120 probes (about 1 eigenvalue here). The designated start 4.0 holds 2.3x the
+    wanted count (138 eigenvalues) and becomes the right end of the search; 1.5 holds fewer
+    than twice the wanted count (55), so the search starts from the spectrum bound: both
+    branches of the start rule run."""
+    n, wanted, margin = 600, 60, 6
+    lam = 1.0 + 59.0 * np.linspace(0.0, 1.0, n) ** 2
+    rng = np.random.default_rng(11)
+    rights = []
+    design = OC.optimize_on_degree
+
+    def spy(*a, **kw):
+        rights.append(float(kw["User_WantedInterval"][1]))
+        return design(*a, **kw)
+
+    monkeypatch.setattr(OC, "optimize_on_degree", spy)
+    pol, cnt, interval = OC.optimize_on_memory(
+        lambda x: lam * x, n, rng, User_MaximumDegree=400, User_MinimumDegree=5,
+        User_WantedInterval=(1.0, 2.0), User_SpectrumBound=(1.0, 60.0 + 1e-12), User_DesignatedStart=designated,
+        User_WantedNumberEigenvalue=wanted, User_AffordableMemoryMargin=margin, User_NumberKpmTrials=120,
+        User_ConvergenceRatio=0.7)
+    assert abs(cnt - wanted) <= margin                      # the loop's own exit test
+    exact = int(np.sum((lam >= interval[0]) & (lam <= interval[1])))
+    assert abs(exact - wanted) <= margin + 3, (exact, interval)
+    assert interval[0] == 1.0 and interval[1] < (designated if designated == 4.0 else 60.0)
+    # the first design is at the designated start; the bisection then starts from [1, 4]
+    # or from [1, 60] (start 1.5, and no start)
+    first_mid = {None: 30.5, 4.0: 2.5, 1.5: 30.5}[designated]
+    expect = [first_mid] if designated is None else [designated, first_mid]
+    assert rights[:len(expect)] == pytest.approx(expect, rel=1e-12)
+    # the returned filter is the design of the returned interval
+    ref = design(400, 5, "Jackson", 1e-10, interval, (1.0, 60.0 + 1e-12), 0.7)
+    assert np.array_equal(np.asarray(pol.AdjustedCoefficients), np.asarray(ref.AdjustedCoefficients))
 
 
 # -------------------------------------------------------------- eigensolvers --
