@@ -32,15 +32,20 @@ __device__ __forceinline__ int64_t cell_coord(double v, double lo, double inv, i
   return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
 }
 
+// Minimum and maximum that propagate NaN (fmin/fmax drop it), so that a NaN coordinate
+// reaches the host's finiteness check.
+__device__ __forceinline__ double nan_min(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
 __global__ void k_minmax(int64_t n, const double *__restrict__ xyz, double *__restrict__ part) {
-  // part[blockIdx.x*6 + {0..2}] = min, {3..5} = max
+  // part[blockIdx.x*6 + {0..2}] = min, {3..5} = max; NaN in either when any coordinate is NaN
   __shared__ double smin[3][256], smax[3][256];
   double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     for (int a = 0; a < 3; ++a) {
       double v = xyz[3 * i + a];
-      mn[a] = fmin(mn[a], v);
-      mx[a] = fmax(mx[a], v);
+      mn[a] = nan_min(mn[a], v);
+      mx[a] = nan_max(mx[a], v);
     }
   }
   for (int a = 0; a < 3; ++a) {
@@ -51,8 +56,8 @@ __global__ void k_minmax(int64_t n, const double *__restrict__ xyz, double *__re
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
       for (int a = 0; a < 3; ++a) {
-        smin[a][threadIdx.x] = fmin(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
-        smax[a][threadIdx.x] = fmax(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+        smin[a][threadIdx.x] = nan_min(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+        smax[a][threadIdx.x] = nan_max(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
       }
     __syncthreads();
   }
@@ -653,11 +658,13 @@ extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, dou
   for (int a = 0; a < 3; ++a) {
     g.lo[a] = INFINITY;
     hi[a] = -INFINITY;
+    bool nan = false;   // std::min/max would drop a NaN partial
     for (int b = 0; b < nb; ++b) {
+      nan |= std::isnan(hp[b * 6 + a]) || std::isnan(hp[b * 6 + 3 + a]);
       g.lo[a] = std::min(g.lo[a], hp[b * 6 + a]);
       hi[a] = std::max(hi[a], hp[b * 6 + 3 + a]);
     }
-    if (!std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_neighbors_celllist: non-finite coordinates");
+    if (nan || !std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_neighbors_celllist: non-finite coordinates");
   }
   g.inv = 1.0 / rc;
   int64_t ncell = 1;
@@ -904,11 +911,13 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   for (int a = 0; a < 3; ++a) {
     g.lo[a] = INFINITY;
     hi[a] = -INFINITY;
+    bool nan = false;   // std::min/max would drop a NaN partial
     for (int b = 0; b < nb; ++b) {
+      nan |= std::isnan(hp[b * 6 + a]) || std::isnan(hp[b * 6 + 3 + a]);
       g.lo[a] = std::min(g.lo[a], hp[b * 6 + a]);
       hi[a] = std::max(hi[a], hp[b * 6 + 3 + a]);
     }
-    if (!std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_anm_clusters: non-finite coordinates");
+    if (nan || !std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_anm_clusters: non-finite coordinates");
   }
   // cells: cell_xy in x and y; cell_z in z, coarsened until there are at most max(2n, 65536) cells
   const double size[3] = {cell_xy, cell_xy, cell_z};
