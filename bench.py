@@ -1,21 +1,29 @@
 #!/usr/bin/env python
 """Time-to-k-modes benchmark on one MI355X.
 
-Workload (default): 300,000-atom seeded globule (0.058 atoms/A^3, 2.2 A minimum
-separation, seed 1), ANM cutoff 8 A, gamma 1, PlusI 1; lowest 64 non-trivial modes at
-relative residual ||A v - lam v|| / (lam_max ||v||) <= 1e-8 (float64 check), rigid-body
-block deflated by a Hotelling shift.
+Workload (default): the 2.4M-atom assembly -- 60 seeded 40,000-atom globular subunits
+(0.058 atoms/A^3, 2.2 A minimum separation, seeds 100..159) rotated by the 60 proper
+rotations of the icosahedron onto an orbit of radius 450 A; ANM cutoff 8 A, gamma 1,
+PlusI 1; the lowest 1000 non-trivial modes at relative residual
+||A v - lam v|| / (lam_max ||v||) <= 1e-8 (float64 check). The subunits are farther apart
+than the cutoff, so H has 60 diagonal blocks: each block's rigid-body modes are deflated and
+Chebyshev-filtered thick-restart Lanczos runs over all 60 blocks in lockstep.
 
-One step = one full solve from coordinates resident in HBM: neighbour search, RCM
-ordering, Hessian assembly, rigid-body basis, spectrum bound, KPM-sized Chebyshev filter,
-Chebyshev-filtered TRLM, float64 polish and residual check. ``value`` is the mean wall
-time per step (seconds, lower is better) of the headline mode; the plain float64 solver
-and the other operator modes are timed in the same run and reported beside it.
+One step = one full solve from coordinates resident in HBM: neighbour search, RCM ordering
+(which also finds the 60 components), cluster layout, spectrum bound, KPM-sized Chebyshev
+filter, filtered lockstep TRLM, float64 check. ``value`` is the mean wall time per step
+(seconds, lower is better) of the headline operator mode; the plain float64 solver (stored
+float64 blocks) and the mixed-precision solve (stored float32 blocks, then float64
+refinement) are timed once each in the same run and reported beside it.
 
-The ``roofline`` object describes the kernel with the largest total device time in the
-timed steps, from HIP events around each of its launches on the launch stream.
-``cpu_baseline`` times the CPU oracle (numpy/scipy restatement of the reference path) on
-a bounded sample of the same workload and scales it to the GPU run's operation counts.
+``roofline`` describes the kernel with the largest total device time in an instrumented
+solve of the headline mode (HIP events on the launch stream around its launches);
+``roofline_block_spmv_f32`` the float32 block SpMV (the stored-value operator of the mixed
+mode) on the same matrix. ``cpu_baseline`` solves one subunit with the CPU oracle and
+ARPACK (the reference's CPU method) and multiplies by the number of subunits.
+
+``--workload globule`` runs the 300,000-atom single-globule line; batch / warm / cpu_plain
+the other configurations.
 """
 import argparse
 import json
@@ -27,25 +35,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--atoms", type=int, default=300000)
     ap.add_argument("--k", type=int, default=64)
-    ap.add_argument("--mode", default="mf64", help="headline operator mode: mixed | mf64 | f64")
+    ap.add_argument("--mode", default="mf64", help="headline operator mode: mf64 | mixed | f64")
     ap.add_argument("--compare", default="f64,mixed", help="other modes timed once each")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--warm-atoms", default="300000,1000000", help="globule sizes of --workload warm")
-    ap.add_argument("--workload", default="globule", choices=["globule", "batch", "assembly", "warm", "cpu_plain"],
-                    help="globule: the headline line; batch / assembly / warm: the other configurations, one "
-                         "JSON line each")
+    ap.add_argument("--workload", default="assembly", choices=["assembly", "globule", "batch", "warm", "cpu_plain"],
+                    help="assembly: the headline line; globule / batch / warm / cpu_plain: the other "
+                         "configurations, one JSON line each")
+    ap.add_argument("--modes", type=int, default=1000, help="modes of the assembly workload")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
                     help="only build the matrix and launch each operator kernel REPS times (profiling runs)")
     return ap.parse_args()
@@ -53,15 +63,19 @@ def parse():
 
 def spmv_only(args):
     """Short program for counter collection: the operator kernels of one solve, launched
-    REPS times each on the workload's matrix, plus the streaming-copy reference kernel."""
+    REPS times each on the workload's matrix (the assembly, or the globule with
+    ``--workload globule``), plus the streaming-copy reference kernel."""
     import torch
 
     from inching_amd import ops, synth
     from inching_amd.hessian import build_hessian
-    X = synth.globule(args.atoms, seed=1)
+    if args.workload == "assembly":
+        X, _ = synth.assembly(n_sub=60, atoms_per_sub=40000, shell_radius=450.0, seed0=100)
+    else:
+        X = synth.globule(args.atoms, seed=1)
     A = build_hessian(X, 8.0, plus_i=1.0, reorder=True, precision=("f32", "f64"))
     n = A.shape[0]
-    out = {"nnzb": A.nnzb, "n_atoms": A.n_atoms}
+    out = {"workload": args.workload, "nnzb": A.nnzb, "n_atoms": A.n_atoms}
     x32 = torch.randn(n, dtype=torch.float32, device="cuda")
     z32, y32, acc32 = torch.randn_like(x32), torch.empty_like(x32), torch.zeros_like(x32)
     x64 = torch.randn(n, dtype=torch.float64, device="cuda")
@@ -128,38 +142,200 @@ def run_batch(args):
                       "lanczos_steps": r.n_spmv}))
 
 
+def _kernel_stats(timer):
+    """Per-kernel totals from HIP event pairs on the launch stream: one pair around each
+    Chebyshev filter application (its operator launches run back to back, no other kernel
+    between them) and one around every other operator launch."""
+    per = {}
+    for kind, epi, e0, e1, launches in timer:
+        per.setdefault((kind, epi), [0.0, 0])
+        per[(kind, epi)][0] += e0.elapsed_time(e1)
+        per[(kind, epi)][1] += launches
+    return per
+
+
+def _roofline(A, kind, epi, ms_total, cnt, step_s, workload):
+    """Roofline object of the operator kernel (kind, epi) of matrix A: algorithmic bytes per
+    launch over its mean launch time, against the HBM peak; for the matrix-free operator
+    also its useful float64 rate (18 flops per stored-equivalent 3x3 block) against the
+    float64 vector peak."""
+    avg_ms = ms_total / cnt
+    by = kernel_bytes(kind, A, epi)
+    ach = by / (avg_ms * 1e-3) / 1e9
+    key = kernel_key(kind, A)
+    out = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(key, workload),
+           "kernel": key + " (" + KERNEL_DESC[kind] + ")", "epilogue": epi, "avg_launch_ms": round(avg_ms, 5),
+           "launches": cnt, "bytes_per_launch": int(by),
+           "kernel_share_of_step": round(ms_total / 1e3 / step_s, 3) if step_s else None}
+    if kind == "mf64":
+        tf = 18.0 * A.nnzb / (avg_ms * 1e-3) / 1e12
+        out["useful_fp64"] = {"achieved_tflops": round(tf, 2), "peak_tflops": FP64_VECTOR_PEAK_TFLOPS,
+                              "frac": round(tf / FP64_VECTOR_PEAK_TFLOPS, 4),
+                              "flops_per_launch": int(18 * A.nnzb),
+                              "valu_issue_frac": measured_valu_issue(key, workload)}
+    return out
+
+
+def _block_spmv_f32_roofline(A, reps, workload):
+    """The tuned float32 block SpMV with the Chebyshev epilogue on matrix A, ``reps``
+    launches bracketed by HIP events on the launch stream."""
+    import torch
+
+    from inching_amd import ops
+    from inching_amd.autotune import best_config, bsr_launch
+    A.ensure("f32")
+    n = A.shape[0]
+    x = torch.randn(n, dtype=torch.float32, device=A.device)
+    z, y, acc = torch.randn_like(x), torch.empty_like(x), torch.zeros_like(x)
+    lanes, c16 = bsr_launch(A, best_config(A, "f32"))
+
+    def launch():
+        ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, A.vals32, x, y, s1=0.1, c0=2.0, z=z, c2=1.0, acc=acc, mu=0.01,
+                     lanes=lanes, col16=c16)
+    for _ in range(3):
+        launch()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    return _roofline(A, "f32", "cheb", e0.elapsed_time(e1), reps, None, workload)
+
+
+def _stream_copy_gbs():
+    import torch
+
+    from inching_amd import ops
+    src = torch.empty(1 << 28, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        ops.stream_copy(src, dst)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        ops.stream_copy(src, dst)
+    e1.record()
+    torch.cuda.synchronize()
+    return 2 * src.numel() * 8 / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
+
+
 def run_assembly(args):
-    """60 x 40,000-atom subunits on an icosahedral orbit (radius 450 A), lowest 1000 modes:
-    the matrix-free float64 operator (headline) and the stored float64 block operator (the
-    plain solver), same filtered lockstep TRLM over the 60 components."""
+    """The headline line: 60 x 40,000-atom subunits on an icosahedral orbit (radius 450 A),
+    lowest ``--modes`` modes at rel_res <= ``--tol``. Headline mode timed over the steps; the
+    plain float64 solver (stored float64 blocks) and the mixed-precision solver (stored
+    float32 blocks, float64 refinement) once each, all with the same filtered lockstep TRLM."""
     import numpy as np
     import torch
 
     from inching_amd import synth
     from inching_amd.batch import solve_assembly
-    xyz, comp = synth.assembly(n_sub=60, atoms_per_sub=40000, shell_radius=450.0, seed0=100)
-    small = np.concatenate([xyz[comp == c] for c in range(3)])
-    solve_assembly(small, 30, per_component=14)           # warm-up
+    dev = torch.device("cuda", 0)
+    X, _ = synth.assembly(n_sub=60, atoms_per_sub=40000, shell_radius=450.0, seed0=100)
+    xyz = torch.as_tensor(X, device=dev)
+    K = args.modes
+
+    def step(mode, timer=None):
+        return solve_assembly(xyz, K, precision=mode, tol_rel=args.tol, timer=timer, seed=0,
+                              verbose=args.verbose)
+
+    for w in range(args.warmup):           # also fills the autotuner's cache for this matrix
+        step(args.mode)
+        print("bench: warmup %d done" % w, file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    out = {"metric": "time-to-1000-modes, 2.4M-atom icosahedral assembly (s)", "unit": "s"}
-    runs = {}
-    for prec in ["mf64"] + [m for m in args.compare.split(",") if m in ("f64", "f32")]:
-        print("bench: assembly %s start" % prec, file=sys.stderr, flush=True)
-        r = solve_assembly(xyz, 1000, precision=prec, tol_rel=1e-8, verbose=args.verbose)
-        runs[prec] = r
-        out[prec] = {"seconds": round(r.seconds, 3), "build_s": round(r.build_seconds, 3), "restarts": r.restarts,
-                     "filter_degree": r.degree, "lanczos_steps": int(r.n_spmv),
-                     "max_rel_res": float(np.max(r.rel_res)), "lam_1000": float(r.eigval[-1])}
-        print("bench: assembly %s done in %.2f s" % (prec, r.seconds), file=sys.stderr, flush=True)
-        del r
+    t0 = time.time()
+    for s_ in range(args.steps):
+        r = step(args.mode)
+        print("bench: step %d done (%.3f s)" % (s_, r.seconds), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    sec = (time.time() - t0) / args.steps
+    A = r.A
+
+    # instrumented solve of the same workload (graphs off, HIP events around operator launches)
+    timer = []
+    torch.cuda.synchronize()
+    ti = time.time()
+    ri = step(args.mode, timer=timer)
+    torch.cuda.synchronize()
+    t_instr = time.time() - ti
+    per = _kernel_stats(timer)
+    (kind, epi), (ms_total, cnt) = max(per.items(), key=lambda kv: kv[1][0])
+    roof = _roofline(ri.A, kind, epi, ms_total, cnt, t_instr, "assembly")
+    kernel_table = {"%s/%s" % kk: {"launches": v[1], "ms_per_step": round(v[0], 3), "avg_ms": round(v[0] / v[1], 5)}
+                    for kk, v in per.items()}
+    print("bench: instrumented solve done", file=sys.stderr, flush=True)
+    del ri, timer
+    summary = {"n_atoms": A.n_atoms, "dof": 3 * A.n_atoms, "nnzb_full": A.nnzb, "components": r.components,
+               "per_component_modes": r.per_component, "rigid_modes_deflated": 6 * r.components,
+               "filter_degree": r.degree, "lanczos_steps": int(r.n_spmv), "restarts": int(r.restarts),
+               "operator_launches": int(r.operator_launches), "max_rel_res": float(np.max(r.rel_res)),
+               "lam_1_minus_plus_i": float(r.eigval[0] - 1.0), "lam_k_minus_plus_i": float(r.eigval[-1] - 1.0),
+               "lam_max_bound": float(r.lam_max)}
+    headline = r
+    del A
+
+    compare = {}
+    roof_block = None
+    for mode in [m for m in args.compare.split(",") if m and m != args.mode]:
         torch.cuda.empty_cache()
-    h = runs["mf64"]
-    out.update({"value": out["mf64"]["seconds"], "components": h.components, "per_component_modes": h.per_component,
-                "rigid_modes_deflated": 6 * h.components, "operator": "mf64"})
-    if "f64" in runs:
-        out["speedup_vs_plain_f64"] = round(out["f64"]["seconds"] / out["mf64"]["seconds"], 3)
-        out["max_abs_dlambda_vs_f64"] = float(np.max(np.abs(runs["f64"].eigval - h.eigval)))
-    print(json.dumps(out))
+        print("bench: compare mode %s start" % mode, file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        ta = time.time()
+        rr = step(mode)
+        torch.cuda.synchronize()
+        compare[mode] = {"seconds": round(time.time() - ta, 3), "max_rel_res": float(np.max(rr.rel_res)),
+                         "lanczos_steps": int(rr.n_spmv), "restarts": int(rr.restarts),
+                         "operator_launches": int(rr.operator_launches), "filter_degree": rr.degree,
+                         "max_abs_dlambda_vs_headline": float(np.max(np.abs(rr.eigval - headline.eigval)))}
+        if rr.phase1_cycles is not None:
+            compare[mode]["float32_phase_cycles"] = int(rr.phase1_cycles)
+        print("bench: compare mode %s done in %.2f s" % (mode, compare[mode]["seconds"]), file=sys.stderr, flush=True)
+        if getattr(rr.A, "vals32", None) is not None and roof_block is None:
+            roof_block = _block_spmv_f32_roofline(rr.A, 20, "assembly")
+        del rr
+    torch.cuda.empty_cache()
+    copy_gbs = _stream_copy_gbs()
+    for rf in (roof, roof_block):
+        if rf is not None:
+            rf["stream_copy_GBs"] = round(copy_gbs, 1)
+            rf["frac_of_stream_copy"] = round(rf["achieved"] / copy_gbs, 4)
+    if "f64" in compare:
+        summary["speedup_vs_plain_f64"] = round(compare["f64"]["seconds"] / sec, 3)
+    if "mixed" in compare:
+        summary["speedup_mixed_vs_plain_f64"] = (round(compare["f64"]["seconds"] / compare["mixed"]["seconds"], 3)
+                                                 if "f64" in compare else None)
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.cpu_baseline import cpu_baseline_assembly
+        print("bench: cpu baseline start", file=sys.stderr, flush=True)
+        cpu = cpu_baseline_assembly(synth.globule(40000, seed=100), headline.components,
+                                    headline.per_component + 6, args.tol, headline.lam_max,
+                                    1.25 * float(np.max(headline.eigval)))
+    line = {
+        "metric": "time-to-k-modes at resid\u22641e-8 (s) + SpMV HBM GB/s; mixed/learned vs plain FP64",
+        "value": round(sec, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": {"mixed": "f32", "f64": "f64", "mf64": "f64", "f32": "f32"}[args.mode],
+        "data": "synthetic (seeded random-sequential-addition globules on an icosahedral orbit, no PDB input)",
+        "config": dict({"workload": "2.4M-atom icosahedral assembly (60 x 40,000-atom subunits, orbit radius "
+                                    "450 A), lowest %d modes at rel_res <= %g, Chebyshev-filtered lockstep TRLM, "
+                                    "operator %s" % (K, args.tol, args.mode)}, **summary),
+        "roofline": roof,
+        "roofline_block_spmv_f32": roof_block,
+        "kernels": kernel_table,
+        "compare_modes": compare,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
 
 
 def run_cpu_plain(args):
@@ -299,10 +475,15 @@ def kernel_key(kind, A):
     return "k_bsr_spmv<%s,%s,%d,int,false>" % (t, t, cfg)
 
 
-def measured_valu_issue(key):
+# committed rocprofv3 counter summaries per workload: (FETCH/WRITE traffic, SQ counters)
+PROFILES = {"globule": ("spmv_kernels_globule300k_pmc_traffic.txt", "matfree_kernels_globule300k_pmc_sq.txt"),
+            "assembly": ("spmv_kernels_assembly2p4M_pmc_traffic.txt", "matfree_kernels_assembly2p4M_pmc_sq.txt")}
+
+
+def measured_valu_issue(key, workload):
     """Share of the chip's VALU issue slots the kernel ``key`` fills, from the committed
     rocprofv3 SQ-counter summary (SQ_INSTS_VALU x 4 cycles / (SIMDs x duration)), or None."""
-    path = os.path.join(ROOT, "profiles", "matfree_kernels_globule300k_pmc_sq.txt")
+    path = os.path.join(ROOT, "profiles", PROFILES[workload][1])
     try:
         with open(path) as f:
             for line in f:
@@ -315,10 +496,10 @@ def measured_valu_issue(key):
     return None
 
 
-def measured_traffic(key):
+def measured_traffic(key, workload):
     """HBM bytes per launch of ``key`` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
     summary (calibrated on a copy kernel), or None when that kernel was not counted."""
-    path = os.path.join(ROOT, "profiles", "spmv_kernels_globule300k_pmc_traffic.txt")
+    path = os.path.join(ROOT, "profiles", PROFILES[workload][0])
     try:
         with open(path) as f:
             for line in f:
@@ -331,17 +512,10 @@ def measured_traffic(key):
     return None
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py: measures one GPU; --gpus %d is not supported (the solve runs on a single GPU)"
-              % args.gpus, file=sys.stderr)
-        sys.exit(2)
-    if args.spmv_only:
-        return spmv_only(args)
-    if args.workload != "globule":
-        return {"batch": run_batch, "assembly": run_assembly, "warm": run_warm,
-                "cpu_plain": run_cpu_plain}[args.workload](args)
+def run_globule(args):
+    """The 300,000-atom single-globule workload (64 modes): one line with the headline
+    mode's time per step, the other modes timed once each, the dominant kernel's roofline
+    and the scaled CPU oracle."""
     import numpy as np
     import torch
 
@@ -400,8 +574,8 @@ def main():
         by = kernel_bytes(kind, A, epi)
         ach = by / (avg_ms * 1e-3) / 1e9
         return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_key(kind, A)),
-                "valu_issue_frac": measured_valu_issue(kernel_key(kind, A)),
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel_key(kind, A), "globule"),
+                "valu_issue_frac": measured_valu_issue(kernel_key(kind, A), "globule"),
                 "kernel": kernel_key(kind, A) + " (" + KERNEL_DESC[kind] + ")",
                 "epilogue": epi, "avg_launch_ms": round(avg_ms, 5), "launches_per_step": cnt // steps,
                 "bytes_per_launch": int(by), "kernel_share_of_step": round(ms_total / steps / 1e3 / step_s, 3)}
@@ -492,6 +666,18 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: measures one GPU; --gpus %d is not supported (the solve runs on a single GPU)"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    if args.spmv_only:
+        return spmv_only(args)
+    return {"assembly": run_assembly, "globule": run_globule, "batch": run_batch, "warm": run_warm,
+            "cpu_plain": run_cpu_plain}[args.workload](args)
 
 
 if __name__ == "__main__":

@@ -79,6 +79,8 @@ SIGNATURES = {
     "inch_gnn_aggregate": (_STATUS, [_i64, _i32, _vp, _vp, _vp, _i32, _vp, _i32, _i32, _vp]),
     "inch_gnn_relu_bwd": (_STATUS, [_i64, _vp, _vp, _vp, _vp]),
     "inch_rr_combine": (_STATUS, [_i64, _i32, _i32, _vp, _i64, _vp, _vp, _i64, _vp]),
+    "inch_sqfluct": (_STATUS, [_i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_cross_correlation": (_STATUS, [_i32, _vp, _i32, _i64, _vp, _vp, _i32, _vp, _vp]),
 }
 
 

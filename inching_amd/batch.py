@@ -8,14 +8,15 @@ connected structure contiguous) and assembly build all of them. Thick-restart La
 then runs every structure in lockstep: the operator is one launch over the whole batch,
 and every inner product, norm, update and restart combination is a *segmented* kernel
 over the structures' dof ranges (``inch_seg_*``; each Lanczos step orthogonalises with the
-segmented delayed CGS2, two reads of the basis). The restart body of all structures is
-captured once into a HIP graph and replayed. The projected eigenproblems (one arrowhead
-matrix per structure) are solved on the host as one batched ``numpy.linalg.eigh``.
+segmented delayed CGS2, two reads of the basis). A restart cycle of all structures is
+captured once into HIP graphs, one per segment of steps, and replayed. The projected
+eigenproblems (one arrowhead matrix per structure) are solved on the host as one batched
+``numpy.linalg.eigh``; between segments they give the Lanczos residual estimates that end
+a cycle as soon as every unconverged structure is predicted to meet its target.
 
-Each structure stops contributing to the result when its Lanczos residual estimate
-meets the tolerance (its latency is recorded then); the batch runs until all have.
-The rigid-body block of every structure is deflated with a Hotelling shift, as in the
-reference's TRLMHD.
+Each structure's latency is recorded when it meets the tolerance; the batch runs until
+all have. The rigid-body block of every structure is deflated with a Hotelling shift, as
+in the reference's TRLMHD.
 """
 import time
 
@@ -177,7 +178,7 @@ def solve_batch(structures, k=20, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=
     torch.cuda.synchronize()
     t0 = time.time()
     X, off = place_apart(structures)
-    prec = {"mf64": (), "f32": ("f32",), "f64": ("f64",)}[precision]
+    prec = {"mf64": (), "f32": ("f32",), "f64": ("f64",), "mixed": ("f32",)}[precision]
     A = build_hessian(X, rc, plus_i=plus_i, reorder=True, precision=prec, device=dev)
     # structures stay contiguous after RCM; locate each by its first atom's new position
     src_of = np.repeat(np.arange(len(structures)), np.diff(off))[A.order]       # structure of each new row
@@ -201,18 +202,68 @@ def components(A):
     return np.asarray(comps, dtype=np.int64)
 
 
+CYCLE_SEGMENT = 16   # lockstep Lanczos steps per captured segment of a cycle (8 with a filter,
+                     # whose steps cost a filter application each)
+STOP_SAFETY = 0.3    # a filtered cycle ends when the calibrated estimate predicts 0.3 x the target
+STALL = 0.7          # a float32 phase ends when its worst residual improves by less than this factor
+
+
+def _projected_ritz(a, b, bk, k, filtered):
+    """Ritz values, eigenvector columns and residual estimates of every structure's projected
+    matrix: tridiagonal alpha / beta ([m, S]), with the arrowhead row beta_k [S, k] after a
+    thick restart (``TRLM/T1.py:320-351``). Largest k for a filtered operator, else smallest k.
+    Returns (Sel [S, k], W [S, m, k], estimates beta_{m-1} W[:, m-1, :] [S, k])."""
+    m, S = a.shape
+    T = np.zeros((S, m, m))
+    idx = np.arange(m)
+    T[:, idx, idx] = a.T
+    T[:, idx[:-1], idx[1:]] = b[:-1].T
+    T[:, idx[1:], idx[:-1]] = b[:-1].T
+    if bk is not None:
+        T[:, k, :k] = bk
+        T[:, :k, k] = bk
+    w, Z = np.linalg.eigh(T)                 # ascending per structure
+    if filtered:
+        Sel, W = w[:, ::-1][:, :k], Z[:, :, ::-1][:, :, :k]
+    else:
+        Sel, W = w[:, :k], Z[:, :, :k]
+    return Sel, W, b[-1][:, None] * W[:, -1, :]
+
+
 def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
-                   factor=2, seed=0, use_graph=True, verbose=False, t0=None, pol=None, ptol=1e-10, dcgs2=True):
+                   factor=2, seed=0, use_graph=True, verbose=False, t0=None, pol=None, dcgs2=True,
+                   segment=None, early_stop=True, phase1_rel=3e-7, v0=None, timer=None):
     """Lockstep thick-restart Lanczos over the contiguous atom segments of a block-diagonal
     A: k modes per segment, per-segment rigid-body Hotelling deflation, optional common
-    Chebyshev filter ``pol`` (largest k of p(A) per segment, then Rayleigh quotients)."""
+    Chebyshev filter ``pol`` (largest k of p(A) per segment, then Rayleigh quotients).
+
+    A restart cycle runs as HIP-graph segments of ``segment`` steps. With ``early_stop`` the
+    host forms every structure's projected matrix from an asynchronous copy of alpha / beta
+    after each segment (while the next one runs) and ends the cycle (a thick restart from the
+    m vectors built) once every unconverged structure's largest Lanczos residual estimate
+    |beta_m s_m,i|, or its geometric extrapolation one segment ahead, meets the threshold:
+    tol_rel * lam_max without a filter (the estimate is the residual of A), and for a filtered
+    operator STOP_SAFETY * tol_rel scaled by the ratio of estimate to float64 residual that the
+    previous cycle's float64 check measured per structure (1 before any check).
+
+    ``precision``: 'mf64' / 'f64' / 'f32' operators throughout, or 'mixed': the float32 block
+    operator until every structure's float64-checked residual reaches ``phase1_rel`` or stops
+    improving, then float64 (matrix-free) filtered subspace steps on the Ritz block with a
+    float64 Rayleigh-Ritz per structure until ``tol_rel``. ``v0``: optional float64 start
+    vector [n] (e.g. a combination of predicted modes); random otherwise. ``timer``: list
+    that receives HIP event pairs around the operator launches (graphs are then off)."""
     dev = A.device
     if t0 is None:
         torch.cuda.synchronize()
         t0 = time.time()
     if seg_struct is None:
         seg_struct = np.arange(len(seg_atoms) - 1)
-    if pol is not None and shift > 0:
+    if precision not in ("mf64", "f64", "f32", "mixed"):
+        raise ValueError("precision must be 'mf64', 'f64', 'f32' or 'mixed'")
+    filtered = pol is not None
+    if segment is None:
+        segment = CYCLE_SEGMENT // 2 if filtered else CYCLE_SEGMENT
+    if filtered and shift > 0:
         shift = -40.0
     S = len(seg_atoms) - 1
     n = A.shape[0]
@@ -224,8 +275,10 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     so = SegOps(seg, max(ncv, 6) + 1, dev)
     dcgs2 = bool(dcgs2) and ncv <= so.dJ
     Q = segmented_rigid_basis(A.xyz, seg_atoms, so)
-    op = Operator(A, precision, pol=pol)
-    filtered = pol is not None
+    mixed = precision == "mixed"
+    op = Operator(A, "f32" if mixed else precision, pol=pol, timer=timer)
+    if timer is not None:
+        use_graph = False
     f64 = dict(dtype=torch.float64, device=dev)
     V = torch.empty((ncv, n), **f64)
     Vk = torch.empty((k, n), **f64)
@@ -237,6 +290,8 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     beta = torch.zeros((ncv, S), **f64)
     scal = torch.empty(S, **f64)
     betak_d = torch.zeros((S, k), **f64)
+    ah = torch.empty((ncv, S), dtype=torch.float64, pin_memory=True)
+    bh = torch.empty((ncv, S), dtype=torch.float64, pin_memory=True)
 
     def apply(x, y):
         op.apply(x, y)
@@ -251,19 +306,18 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
             if dcgs2:
                 # delayed CGS2: V[i] is corrected here, its successor at the next step
                 so.dcgs2(V, J, u, alpha[i], beta[i - 1] if i > 0 else None, beta[i])
-                if i < i1 - 1:
-                    so.scale(u, beta[i], V[i + 1])
-                continue
-            so.multidot(V, J, u, h)
-            alpha[i].copy_(h[:S * J].view(S, J)[:, i])
-            # CGS2: update + second projection in one read of V (beta from the first update),
-            # then the second update
-            so.cgs_b(V, J, h, J, u, h2, beta[i])
-            so.update(V, J, h2, J + 1, 1.0, u)
-            if i < i1 - 1:
+            else:
+                so.multidot(V, J, u, h)
+                alpha[i].copy_(h[:S * J].view(S, J)[:, i])
+                # CGS2: update + second projection in one read of V (beta from the first
+                # update), then the second update
+                so.cgs_b(V, J, h, J, u, h2, beta[i])
+                so.update(V, J, h2, J + 1, 1.0, u)
+            if i + 1 < ncv:
                 so.scale(u, beta[i], V[i + 1])
 
-    def body():
+    def head():
+        # restart vector orthogonalised against the k Ritz vectors, then the arrowhead step
         so.multidot(V, k, u, h)
         so.update(V, k, h, k, 1.0, u, nrm=scal)
         so.scale(u, scal, V[k])
@@ -272,112 +326,249 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
         so.update(V[k:k + 1], 1, alpha[k], 1, 1.0, u)
         so.update(V, k, betak_d, k, 1.0, u, nrm=beta[k])
         so.scale(u, beta[k], V[k + 1])
-        lanczos(k + 1, ncv)
-        finish()
 
-    def finish():
+    graphs = {}
+
+    def run_segment(first, s, lo, hi):
+        key = (s, op.precision)
+        if not use_graph or first:          # the first sweep runs once: launched directly
+            if not first and s == 0:
+                head()
+            lanczos(lo, hi)
+            return
+        if key not in graphs:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                if not first and s == 0:
+                    head()
+                lanczos(lo, hi)
+            graphs[key] = g
+        graphs[key].replay()
+
+    def finish(m):
         # with delayed CGS2 the last residual has had one pass: give it the second
         if dcgs2:
-            so.multidot(V, ncv, u, h)
-            so.update(V, ncv, h, ncv, 1.0, u)
+            so.multidot(V, m, u, h)
+            so.update(V, m, h, m, 1.0, u)
 
-    def restart(a, b, bk):
-        T = np.zeros((S, ncv, ncv))
-        idx = np.arange(ncv)
-        T[:, idx, idx] = a.T
-        T[:, idx[:-1], idx[1:]] = b[:-1].T
-        T[:, idx[1:], idx[:-1]] = b[:-1].T
-        if bk is not None:
-            T[:, k, :k] = bk
-            T[:, :k, k] = bk
-        w, Z = np.linalg.eigh(T)                 # ascending per structure
-        if filtered:
-            Sel, W = w[:, ::-1][:, :k], Z[:, :, ::-1][:, :, :k]
-        else:
-            Sel, W = w[:, :k], Z[:, :, :k]
+    steps = [0]
+
+    def run_cycle(first, Sel, bk, thr, active):
+        """One cycle (the first Lanczos sweep, or a thick-restart cycle from the k Ritz
+        vectors in V[:k] and the residual in u); returns the number m of basis vectors built."""
+        lo0 = 0 if first else k + 1
+        b_ = list(range(lo0, ncv, segment)) + [ncv]
+        pending, prev = None, None
+        for s, (lo, hi) in enumerate(zip(b_[:-1], b_[1:])):
+            run_segment(first, s, lo, hi)
+            steps[0] += hi - lo + (0 if first or s else 1)
+            if hi >= ncv:
+                return ncv
+            if thr is None:
+                continue
+            if pending is not None and pending[0] >= k + 2:
+                m, ev = pending
+                ev.synchronize()
+                a, b = ah[:m].numpy().copy(), bh[:m].numpy().copy()
+                if not first:
+                    a[:k] = Sel.T
+                    b[:k] = 0.0
+                est = np.abs(_projected_ritz(a[:, active], b[:, active], None if first else bk[active], k,
+                                             filtered)[2]).max(1)
+                th = thr[active]
+                ok = est <= th
+                if prev is not None:
+                    ok |= (est < prev) & (est * est <= 0.3 * th * prev)
+                if ok.all():
+                    return hi
+                prev = est
+            ah[:hi].copy_(alpha[:hi], non_blocking=True)
+            bh[:hi].copy_(beta[:hi], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            pending = (hi, ev)
+        return ncv
+
+    def restart(m, Sel, bk, first):
+        a, b = alpha[:m].cpu().numpy().copy(), beta[:m].cpu().numpy().copy()
+        if not first:
+            a[:k] = Sel.T
+            b[:k] = 0.0
+        Sel, W, bkn = _projected_ritz(a, b, None if first else bk, k, filtered)
         Wd = torch.as_tensor(np.ascontiguousarray(W), device=dev)
-        so.combine(V, ncv, Wd, Vk)
+        so.combine(V, m, Wd, Vk)
         V[:k].copy_(Vk)
-        bkn = b[-1][:, None] * W[:, -1, :]       # [S, k]
         return Sel, bkn
 
-    g = torch.Generator(device=dev)
-    g.manual_seed(int(seed))
-    u.copy_(torch.rand(n, generator=g, dtype=torch.float64, device=dev))
+    def verify(X):
+        """float64 check of the row block X [k, n]: per-structure eigenvalue estimates
+        (Ritz values without a filter, Rayleigh quotients with one) and relative residuals."""
+        Y = A.matfree(X.T.contiguous()).T.contiguous()
+        rq = torch.empty((k, S), **f64)
+        for q in range(k):
+            so.multidot(X[q:q + 1], 1, Y[q], rq[q])
+        theta_h = rq.T.cpu().numpy()                # [S, k]
+        rn = torch.empty(S * k, **f64)
+        so.resnorm(X, Y, torch.as_tensor(np.ascontiguousarray(theta_h), device=dev), rn)
+        return theta_h, rn.cpu().numpy().reshape(S, k) / lam_max
+
+    # lam_max for the relative residual: a Lanczos bound of the whole batch (the filter's
+    # upper end when one is given)
+    from .chebyshev import BOUND_STOP_REL, spectrum_bound
+    lam_max = (spectrum_bound(A, precision="f32" if mixed else precision, stop_rel=BOUND_STOP_REL)[1]
+               if pol is None else float(pol.cc + pol.dd))
+    tol_abs = tol_rel * lam_max
+    target = phase1_rel if mixed else tol_rel
+    # in-cycle thresholds per structure, in the units of the Lanczos estimate
+    thr = np.full(S, (target * lam_max) if not filtered else STOP_SAFETY * target)
+
+    if v0 is None:
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(seed))
+        u.copy_(torch.rand(n, generator=g, dtype=torch.float64, device=dev))
+    else:
+        u.copy_(torch.as_tensor(v0, dtype=torch.float64, device=dev).reshape(-1))
     so.update(V, 0, None, 1, 0.0, u, nrm=scal)
     so.scale(u, scal, V[0])
-    lanczos(0, ncv)
-    finish()
-    a, b = alpha.cpu().numpy(), beta.cpu().numpy()
-    Sel, bk = restart(a, b, None)
-    lam_max = None
     done_at = np.full(S, np.nan)
-    res = np.linalg.norm(bk, axis=1)
-    graph = None
-    it = 0
-    # lam_max for the relative residual: a 300-step Lanczos bound of the whole batch
-    from .chebyshev import BOUND_STOP_REL, spectrum_bound
-    lam_max = (spectrum_bound(A, precision=precision, stop_rel=BOUND_STOP_REL)[1] if pol is None
-               else float(pol.cc + pol.dd))
-    tol_abs = tol_rel * lam_max if pol is None else ptol * np.sqrt(k)
-    conv = res <= tol_abs
-    done_at[conv] = time.time() - t0
-    cycles = 0                                   # restart cycles that ran (body() launches)
-    for it in range(maxiter):
-        if conv.all():
-            break
-        betak_d.copy_(torch.as_tensor(bk, device=dev))
-        if use_graph:
-            if graph is None:
-                graph = torch.cuda.CUDAGraph()
-                with torch.cuda.graph(graph):
-                    body()
-            graph.replay()
+    conv = np.zeros(S, dtype=bool)
+    checked = None          # (theta [S, k], rel [S, k]) of the last float64 check of V[:k]
+    hist = []
+    Sel, bk = None, None
+    cycles = 0              # restart cycles after the first sweep
+    phase1_cycles = None
+    for it in range(maxiter + 1):
+        first = it == 0
+        active = ~conv
+        # in-cycle checks: always for a filtered operator; without a filter from the second
+        # cycle on, once every unconverged structure's estimate is within reach (the host
+        # eigh of every projected matrix per segment is not free for large batches)
+        use_thr = early_stop and (filtered or (not first and bool(np.all(est[active] <= 1e4 * thr[active]))))
+        if not first:
+            betak_d.copy_(torch.as_tensor(bk, device=dev))
+        m = run_cycle(first, Sel, bk, thr if use_thr else None, active)
+        if not first:
+            cycles += 1
+        finish(m)
+        Sel, bk = restart(m, Sel, bk, first)
+        est = np.abs(bk).max(1)                     # [S]
+        if not filtered and not mixed:
+            newly = active & (est <= tol_abs)
         else:
-            body()
-        cycles += 1
-        a, b = alpha.cpu().numpy().copy(), beta.cpu().numpy().copy()
-        a[:k] = Sel.T
-        b[:k] = 0.0
-        Sel, bk = restart(a, b, bk)
-        res = np.linalg.norm(bk, axis=1)
-        newly = (~conv) & (res <= tol_abs)
+            checked = verify(V[:k])
+            worst = checked[1].max(1)
+            newly = active & (worst <= target)
+            # calibrate the in-cycle threshold: estimate per unit of float64 residual
+            thr = np.where(worst > 0, STOP_SAFETY * target * est / np.maximum(worst, 1e-300), thr)
+            hist.append(float(worst.max()))
         done_at[newly] = time.time() - t0
         conv |= newly
         if verbose:
-            print("batch restart %d: %d/%d converged, max est %.2e" % (it, conv.sum(), S, res.max()))
-    # float64 verification of every pair (matrix-free operator, segmented residual norms)
+            print("lockstep cycle %d (m=%d): %d/%d converged, max estimate %.2e%s"
+                  % (it, m, conv.sum(), S, est.max(), "" if checked is None else ", worst float64 rel_res %.2e"
+                     % checked[1].max()))
+        if conv.all():
+            break
+        if mixed and len(hist) >= 2 and hist[-1] > STALL * hist[-2]:
+            break                                   # the float32 operator's floor
+    if mixed:
+        phase1_cycles = cycles
+        if filtered:
+            checked = _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, verbose=verbose,
+                                         timer=timer)
+        else:
+            # no filter to refine with: a float64 lockstep run started from a combination of
+            # every structure's float32 Ritz vectors
+            gc = torch.Generator(device=dev)
+            gc.manual_seed(int(seed) + 1)
+            c = torch.rand(k, generator=gc, dtype=torch.float64, device=dev) + 0.5
+            r2 = solve_segments(A, seg_atoms, k, seg_struct=seg_struct, precision="mf64", tol_rel=tol_rel,
+                                shift=shift, maxiter=maxiter, factor=factor, seed=seed, use_graph=use_graph,
+                                verbose=verbose, t0=t0, dcgs2=dcgs2, segment=segment, early_stop=early_stop,
+                                v0=(c[:, None] * V[:k]).sum(0), timer=timer)
+            r2.phase1_cycles = phase1_cycles
+            r2.restarts += cycles
+            r2.n_spmv += steps[0]
+            r2.operator_launches += op.n_spmv
+            return r2
+        now = time.time() - t0
+        conv = checked[1].max(1) <= tol_rel
+        done_at[:] = now
+    # float64 check of every pair (matrix-free operator, segmented residual norms)
     Xr = V[:k]
-    Yc = A.matfree(Xr.T.contiguous())
-    Y = Yc.T.contiguous()
-    if filtered:
-        # Rayleigh quotients with A per segment and mode (the Ritz values above are of p(A))
-        rq = torch.empty((k, S), **f64)
-        for q in range(k):
-            so.multidot(Xr[q:q + 1], 1, Y[q], rq[q])
-        Sel = rq.T.cpu().numpy()                  # [S, k], paired with the rows of Xr
-    theta = torch.as_tensor(np.ascontiguousarray(Sel), device=dev)
-    rn = torch.empty(S * k, **f64)
-    so.resnorm(Xr, Y, theta, rn)
+    if checked is None:
+        if filtered:
+            checked = verify(Xr)
+        else:
+            # Ritz values of A itself; residuals with the matrix-free float64 operator
+            Y = A.matfree(Xr.T.contiguous()).T.contiguous()
+            rn = torch.empty(S * k, **f64)
+            so.resnorm(Xr, Y, torch.as_tensor(np.ascontiguousarray(Sel), device=dev), rn)
+            checked = (Sel, rn.cpu().numpy().reshape(S, k) / lam_max)
+    theta_h, rel = checked
     torch.cuda.synchronize()
     total = time.time() - t0
-    rel = rn.cpu().numpy().reshape(S, k) / lam_max
-    if filtered:
-        lam_max = float(pol.cc + pol.dd)
-        rel = rn.cpu().numpy().reshape(S, k) / lam_max
-        ordq = np.argsort(Sel, axis=1)             # ascending eigenvalues per segment
-        Sel = np.take_along_axis(Sel, ordq, axis=1)
-        rel = np.take_along_axis(rel, ordq, axis=1)
+    ordq = np.argsort(theta_h, axis=1, kind="stable")   # ascending eigenvalues per segment
+    theta_h = np.take_along_axis(theta_h, ordq, axis=1)
+    rel = np.take_along_axis(rel, ordq, axis=1)
     order = np.argsort(seg_struct)
     done = np.where(np.isnan(done_at), total, done_at)
-    n_steps = ncv + cycles * (ncv - k)          # operator applications of the lockstep Lanczos
-    return BatchResult(eigval=Sel[order], rel_res=rel[order], latency=done[order], seconds=total,
-                       restarts=cycles, n_spmv=n_steps, ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
-                       seg_struct=seg_struct, A=A, eigvec=Xr, converged=int(conv.sum()))
+    return BatchResult(eigval=theta_h[order], rel_res=rel[order], latency=done[order], seconds=total,
+                       restarts=cycles, n_spmv=steps[0], ncv=ncv, lam_max=lam_max, seg_atoms=seg_atoms,
+                       seg_struct=seg_struct, A=A, eigvec=Xr, mode_order=ordq[order], converged=int(conv.sum()),
+                       operator_launches=op.n_spmv, phase1_cycles=phase1_cycles)
+
+
+def _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, maxiter=4, verbose=False, timer=None):
+    """Float64 refinement of the Ritz block V[:k] of a float32 phase: filtered subspace steps
+    X <- p(X) with the matrix-free float64 operator (the rigid-body rows projected out per
+    structure after each filter), then a float64 Rayleigh-Ritz per structure (Cholesky of the
+    segmented Gram matrix, eigh of the projected pencil) and the residuals, until every pair
+    meets ``tol_rel``. ``checked``: (theta, rel) of the block as it is; ``verify(X)`` gives
+    them for a block X. Returns (theta [S, k], rel [S, k]); V[:k] holds the refined vectors."""
+    dev = V.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    op = Operator(A, "mf64", pol=pol, timer=timer)
+    n = V.shape[1]
+    X = V[:k]
+    Y = torch.empty((k, n), **f64)
+    hq = torch.empty(S * 6, **f64)
+    h = torch.empty(S * k, **f64)
+    theta_h, rel = checked
+    for it in range(maxiter):
+        if rel.max() <= tol_rel:
+            break
+        for q in range(k):
+            op.apply(X[q], Y[q])
+            for _ in range(2):
+                so.multidot(Q, 6, Y[q], hq)
+                so.update(Q, 6, hq, 6, 1.0, Y[q])
+        Z = A.matfree(Y.T.contiguous()).T.contiguous()
+        G = np.empty((S, k, k))
+        H = np.empty((S, k, k))
+        for q in range(k):
+            so.multidot(Y, k, Y[q], h)
+            G[:, :, q] = h.view(S, k).cpu().numpy()
+            so.multidot(Y, k, Z[q], h)
+            H[:, :, q] = h.view(S, k).cpu().numpy()
+        G = 0.5 * (G + G.transpose(0, 2, 1))
+        H = 0.5 * (H + H.transpose(0, 2, 1))
+        L = np.linalg.cholesky(G)
+        Li = np.linalg.inv(L)
+        C = Li @ H @ Li.transpose(0, 2, 1)
+        th, U = np.linalg.eigh(0.5 * (C + C.transpose(0, 2, 1)))
+        W = np.ascontiguousarray(Li.transpose(0, 2, 1) @ U)      # [S, k, k]: X_new = W^T Y per structure
+        Wd = torch.as_tensor(W, device=dev)
+        so.combine(Y, k, Wd, X)
+        theta_h, rel = verify(X)
+        if verbose:
+            print("float64 block refinement %d: worst rel_res %.2e" % (it, rel.max()))
+    return theta_h, rel
 
 
 def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, per_component=None,
-                   filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False, factor=3, retries=2):
+                   filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False, factor=3, retries=2,
+                   timer=None, v0=None):
     """Lowest ``k_total`` non-trivial modes of a structure made of disconnected components
     (e.g. subunits of an assembly farther apart than the cutoff). H is block-diagonal, so its
     spectrum is the union of the components' spectra: every component is solved for
@@ -385,19 +576,27 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
     its own six rigid-body modes deflated), the results are merged, and the merge is checked:
     a component whose highest computed mode lies below the k_total-th merged eigenvalue could
     hide more modes below it. Then the solve is repeated with twice the modes per component
-    (at most ``retries`` times), after which it raises."""
+    (at most ``retries`` times), after which it raises.
+
+    ``precision``: 'mf64' (matrix-free float64), 'f64' (stored float64 blocks: the plain
+    float64 solver), 'f32' (stored float32 blocks, no float64 phase) or 'mixed' (float32
+    blocks for the bound, the filter design and the Lanczos phase, then float64 refinement of
+    the Ritz block). ``v0``: optional start vector [3N] in the RCM order of the build, or a
+    callable A -> start vector (e.g. a learned warm start)."""
     from . import chebyshev as CH
     ops.require_gpu()
     dev = torch.device("cuda", torch.cuda.current_device())
     torch.cuda.synchronize()
     t0 = time.time()
-    prec = {"mf64": (), "f32": ("f32",), "f64": ("f64",)}[precision]
+    prec = {"mf64": (), "f32": ("f32",), "f64": ("f64",), "mixed": ("f32",)}[precision]
+    bulk = "f32" if precision == "mixed" else precision       # operator of the bound and the KPM probes
     A = build_hessian(xyz, rc, plus_i=plus_i, reorder=True, precision=prec, device=dev)
     seg_atoms = components(A)
     C = len(seg_atoms) - 1
     t_build = time.time() - t0
     ks = int(per_component) if per_component else int(np.ceil(1.15 * k_total / C)) + 4
-    lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL) if filtered else (None, None)
+    lo, hi = CH.spectrum_bound(A, precision=bulk, stop_rel=CH.BOUND_STOP_REL) if filtered else (None, None)
+    start = v0(A, seg_atoms) if callable(v0) else v0
     for attempt in range(retries + 1):
         pol = None
         if filtered:
@@ -405,9 +604,9 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
             pol, cnt, interval, _ = CH.optimize_on_memory(A, bound, A.plus_i, wanted_count=int(C * (ks + 6) * 1.5),
                                                           designated_start=bound[1] / 10.0, max_degree=max_degree,
                                                           min_degree=5, margin=max(5, C * ks // 10), trials=5,
-                                                          ratio=0.7, seed=seed, precision=precision)
+                                                          ratio=0.7, seed=seed, precision=bulk)
         r = solve_segments(A, seg_atoms, ks, precision=precision, tol_rel=tol_rel, maxiter=maxiter, seed=seed,
-                           factor=factor, verbose=verbose, t0=t0, pol=pol)
+                           factor=factor, verbose=verbose, t0=t0, pol=pol, timer=timer, v0=start)
         flat = r.eigval.reshape(-1)
         order = np.argsort(flat, kind="stable")[:k_total]
         cutoff = flat[order[-1]]
@@ -425,4 +624,5 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
                        rel_res=r.rel_res.reshape(-1)[order], seconds=time.time() - t0, build_seconds=t_build,
                        components=C, per_component=ks, restarts=r.restarts, n_spmv=r.n_spmv,
                        degree=int(pol.AdjustedDegree) if pol is not None else 0, A=A, seg_atoms=seg_atoms,
-                       eigvec=r.eigvec, lam_max=r.lam_max)
+                       eigvec=r.eigvec, lam_max=r.lam_max, operator_launches=r.operator_launches,
+                       phase1_cycles=r.phase1_cycles, precision=precision)

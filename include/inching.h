@@ -360,6 +360,23 @@ inch_status inch_seg_resnorm(int32_t nchunk, int32_t S, const int64_t *cbeg, con
                              const int64_t *sptr, int32_t k, const double *X, int64_t ldx, const double *Y,
                              int64_t ldy, const double *theta, double *out, double *work, void *stream);
 
+/* ------------------------------------------------- analysis of computed modes ---- */
+/* Square fluctuations of the modes eigvec [k, N, 3] (float64, atom order of the caller):
+ * ratio_var[m] = r_m / max r, r_m = 1 / sqrt(|eigval[m]|); sfpm[m, a] = |eigvec[m, a, :]|;
+ * ratio[m, a] = ratio_var[m] sfpm[m, a]; sqfluct[a] = sum_m ratio[m, a]. eigval is a device
+ * pointer. Replaces Heigval_Heigvec_SqFluctPerMode_SqFluct_RatioSqFPM_RatioVar,
+ * Burn/Visualisation/T2.py:43-55. */
+inch_status inch_sqfluct(int32_t k, int64_t N, const double *eigval, const double *eigvec, double *ratio_var,
+                         double *sfpm, double *ratio, double *sqfluct, void *stream);
+/* Cross-correlation maps of P pairs of modes: out[p, a, b] (float32 [P, N, N]) =
+ * sign(s) log2(|s| + 1), s = float32((eigvec[i, a, :] . eigvec[j, b, :]) eigval[i] eigval[j])
+ * * 100^precision, (i, j) = pairs[2p], pairs[2p+1]. pairs (int32 [P, 2]) and eigval
+ * (float64 [k]) are HOST pointers; every index is checked against k; precision in [0, 4].
+ * Replaces the hessian_CC loop and S_LogModulusS of Heigval_Heigvec_HccPairlist_HccBatch,
+ * Burn/Visualisation/T2.py:58-116 and Burn/Visualisation/T1.py:44-52. */
+inch_status inch_cross_correlation(int32_t P, const int32_t *pairs, int32_t k, int64_t N, const double *eigval,
+                                   const double *eigvec, int32_t precision, float *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

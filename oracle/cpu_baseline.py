@@ -1,4 +1,7 @@
-"""CPU baseline of the time-to-k-modes workload, timed with the oracle (numpy/scipy).
+"""CPU baselines of the time-to-k-modes workloads, timed with the oracle (numpy/scipy).
+
+``cpu_baseline_assembly`` (the disconnected assembly) is described in its docstring; the
+rest of this header describes ``cpu_baseline`` (one connected globule).
 
 The reference's CPU-side path is its own Python/numpy code (3DRCM loop over k-d tree
 queries, ``InchingLiteInteger/Fuel/Coordinate/T1.py:171-323``) plus an FP64 L+D CSR
@@ -71,3 +74,52 @@ def cpu_baseline(X, full_atoms, full_nnzb, k, gpu_result):
             % (sample_atoms, t_nb, t_rcm, t_h, t_spmv, scale_n, scale_b, gpu_result.total_spmv, n, j, t_cgs,
                gpu_result.n_apply, cores),
             "breakdown_s": {"setup": round(setup, 2), "spmv": round(spmv, 2), "reorth": round(cgs, 2)}}
+
+
+def cpu_baseline_assembly(X_sub, n_components, k_with_rigid, tol_rel, lam_max, theta_max, rc=8.0, plus_i=1.0):
+    """CPU time of the disconnected-assembly workload by the reference's CPU method, timed on
+    one of its subunits and multiplied by the number of subunits.
+
+    The assembly's Hessian is block-diagonal with one block per subunit, so its lowest modes
+    are the union of the subunits' lowest modes and a CPU solve of the assembly is a solve
+    of each subunit. The sample is one subunit ``X_sub`` (coordinates [N_sub, 3]; a rotation
+    does not change its spectrum), taken end to end: neighbours, 3DRCM and the CSR Hessian
+    with the oracle (``Fuel/Coordinate/T1.py:171-323``, ``Fuel/T1.py:228-660``), then
+    ``scipy.sparse.linalg.eigsh(H + PlusI I, k, which='SA')``, the ARPACK call of the
+    reference's CPU baseline (``Command8A/commandArpack_Pdb_0064_EPYC.py:328``), with
+    ``k_with_rigid`` modes (the subunit's wanted modes and its six rigid-body modes).
+    ARPACK stops at ||r|| <= tol |theta|; tol = tol_rel lam_max / theta_max makes that the
+    relative-residual target of the GPU run (``theta_max``: its largest wanted eigenvalue).
+    The residuals of the CPU result are checked against tol_rel afterwards."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+
+    cores = len(os.sched_getaffinity(0))
+    X = np.asarray(X_sub, dtype=np.float64)
+    N = X.shape[0]
+    t0 = time.time()
+    ip, ix = OG.neighbour_csr(X, rc)
+    t_nb = time.time() - t0
+    t = time.time()
+    order, _ = OR.rcm_order(ip, ix)
+    t_rcm = time.time() - t
+    t = time.time()
+    Xp = X[order]
+    rp, col, v = OH.block_hessian(Xp, rc, plus_i)
+    H = sp.bsr_matrix((v, col.astype(np.int64), rp), shape=(3 * N, 3 * N)).tocsr()
+    t_h = time.time() - t
+    tol = tol_rel * lam_max / theta_max
+    t = time.time()
+    w, U = sla.eigsh(H, k=k_with_rigid, which="SA", tol=tol)
+    t_eig = time.time() - t
+    rel = np.linalg.norm(H @ U - U * w[None, :], axis=0) / lam_max
+    per_sub = time.time() - t0
+    return {"value": round(per_sub * n_components, 1), "unit": "s", "cores": cores, "kind": "port",
+            "sample": ("one of the %d subunits (%d atoms, the first subunit's globule) solved completely on the "
+                       "host: oracle neighbours %.2fs + 3DRCM %.2fs + CSR Hessian %.2fs, then scipy eigsh(k=%d, "
+                       "which='SA', tol=%.2e) %.2fs (ARPACK, the reference's CPU method; matvec single-threaded, "
+                       "BLAS on %d cores); max rel_res %.1e; value = %d x %.2fs")
+            % (n_components, N, t_nb, t_rcm, t_h, k_with_rigid, tol, t_eig, cores, float(rel.max()),
+               n_components, per_sub),
+            "per_subunit_s": round(per_sub, 3), "max_rel_res": float(rel.max()),
+            "lowest_nontrivial": [float(x) for x in np.sort(w)[6:9]]}
