@@ -63,6 +63,7 @@ SIGNATURES = {
     "inch_cgs2_reorth": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "inch_dcgs2_work_size": (_i64, [_i32]),
     "inch_dcgs2_step": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_dcgs2_step_noalloc": (_STATUS, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "inch_seg_multidot": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
     "inch_seg_update": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, _dbl, _vp, _vp, _vp,
                                   _vp]),

@@ -11,6 +11,8 @@
 // group apply the epilogue to the row's 3 dofs.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "common.h"
 
 #define HIP_TRY(expr)                                                                      \
@@ -1236,11 +1238,16 @@ inch_status inch_anm_matfree_cl_apply(int64_t n, const double *xyz, int64_t ncl,
 
 int64_t inch_dcgs2_work_size(int32_t J) { return (int64_t)kRedBlocks * (2 * (int64_t)J + 1) + 4 * (int64_t)J; }
 
-inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
-                            const double *beta, double *beta_out, double *vnext, double *work, void *stream) {
-  INCH_REQUIRE(len > 0 && J >= 1 && V && u && alpha && beta_out && work && ld >= len, "inch_dcgs2_step: bad args");
-  INCH_REQUIRE(J == 1 || beta, "inch_dcgs2_step: beta needed for J > 1");
-  hipStream_t st = (hipStream_t)stream;
+}  // extern "C"
+
+namespace {
+
+// The kernels of one delayed-CGS2 step on the scratch `work` (inch_dcgs2_work_size(J) doubles:
+// per-block partial dots, the reduced dots, the coefficients, the norm partials). Every word
+// of `work` that a kernel reads was written earlier in the same step, so the scratch needs no
+// clearing between steps.
+inch_status dcgs2_launch(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
+                         const double *beta, double *beta_out, double *vnext, double *work, hipStream_t st) {
   constexpr int JT = 8;
   const int gx = grid_for(len, 256, kRedBlocks);
   double *part = work;
@@ -1266,6 +1273,58 @@ inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, doubl
     HIP_TRY(hipGetLastError());
   }
   return INCH_OK;
+}
+
+// Static scratch of the allocation-free step: one copy per GPU, sized for J <= kNoAllocMaxJ
+// (2.0 MiB), held from the library's load until the process ends. Host code never passes the
+// array's name to a kernel: its device address comes from hipGetSymbolAddress, once per
+// device (with that device current), and is kept per device.
+constexpr int32_t kNoAllocMaxJ = 256;
+constexpr int64_t kNoAllocWords = (int64_t)kRedBlocks * (2 * kNoAllocMaxJ + 1) + 4 * kNoAllocMaxJ;
+static __device__ __align__(256) double scratch[kNoAllocWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+inch_status scratch_for_current_device(double **out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) INCH_FAIL(INCH_ERR_INVALID, "scratch: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      INCH_FAIL(INCH_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return INCH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
+                            const double *beta, double *beta_out, double *vnext, double *work, void *stream) {
+  INCH_REQUIRE(len > 0 && J >= 1 && V && u && alpha && beta_out && work && ld >= len, "inch_dcgs2_step: bad args");
+  INCH_REQUIRE(J == 1 || beta, "inch_dcgs2_step: beta needed for J > 1");
+  return dcgs2_launch(len, J, V, ld, u, alpha, beta, beta_out, vnext, work, (hipStream_t)stream);
+}
+
+inch_status inch_dcgs2_step_noalloc(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
+                                    const double *beta, double *beta_out, double *vnext, void *stream) {
+  INCH_REQUIRE(len > 0 && J >= 1 && V && u && alpha && beta_out && ld >= len, "inch_dcgs2_step_noalloc: bad args");
+  INCH_REQUIRE(J == 1 || beta, "inch_dcgs2_step_noalloc: beta needed for J > 1");
+  if (J > kNoAllocMaxJ)
+    INCH_FAIL(INCH_ERR_CAPACITY, "inch_dcgs2_step_noalloc: J = %d basis vectors exceed the %d its static scratch holds",
+              J, kNoAllocMaxJ);
+  double *work = nullptr;
+  const inch_status s = scratch_for_current_device(&work);
+  if (s != INCH_OK) return s;
+  return dcgs2_launch(len, J, V, ld, u, alpha, beta, beta_out, vnext, work, (hipStream_t)stream);
 }
 
 inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *stream) {

@@ -293,6 +293,16 @@ inch_status inch_cgs2_reorth(int64_t len, int32_t J, const double *V, int64_t ld
 int64_t inch_dcgs2_work_size(int32_t J);
 inch_status inch_dcgs2_step(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
                             const double *beta, double *beta_out, double *vnext, double *work, void *stream);
+/* Allocation-free form of inch_dcgs2_step: the same kernels and results, with the step's
+ * scratch in a static device array of the library instead of `work` (J <= 256; larger J
+ * returns INCH_ERR_CAPACITY). No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (HIP graph capture, serving loops). It allocates nothing; the array
+ * (2.0 MiB per GPU) is held until the process ends, and its device address is looked up once
+ * per device on the first call there. Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (other streams, other host threads) race. No word of the array
+ * needs clearing between calls. */
+inch_status inch_dcgs2_step_noalloc(int64_t len, int32_t J, double *V, int64_t ld, double *u, double *alpha,
+                                    const double *beta, double *beta_out, double *vnext, void *stream);
 
 /* out[q*ld_out + i] = sum_r W[r*k + q] * V[r*ld + i]  for q < k, r < m
  * (W float64 [m, k] row-major on device; out must not alias V). */
