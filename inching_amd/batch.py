@@ -269,7 +269,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     n = A.shape[0]
     seg = Segments(3 * seg_atoms, dev)
     min_dof = int(seg.sizes.min())
-    ncv = min(max(factor * k, k + 32), min_dof - 7)
+    ncv = min(max(int(round(factor * k)), k + 32), min_dof - 7)
     if ncv <= k + 1:
         raise AssertionError("ABORTED. k must be smaller than every structure's dof count minus the rigid modes")
     so = SegOps(seg, max(ncv, 6) + 1, dev)
@@ -379,6 +379,9 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
                 ok = est <= th
                 if prev is not None:
                     ok |= (est < prev) & (est * est <= 0.3 * th * prev)
+                if verbose:
+                    print("  m=%d: worst estimate / threshold %.2e, %d of %d structures ready"
+                          % (m, float(np.max(est / th)), int(ok.sum()), ok.size))
                 if ok.all():
                     return hi
                 prev = est
@@ -568,7 +571,7 @@ def _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, maxiter
 
 def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, per_component=None,
                    filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False, factor=3, retries=2,
-                   timer=None, v0=None):
+                   timer=None, v0=None, kpm_factor=1.5, ratio=0.7, kpm_trials=4, filter_right=None):
     """Lowest ``k_total`` non-trivial modes of a structure made of disconnected components
     (e.g. subunits of an assembly farther apart than the cutoff). H is block-diagonal, so its
     spectrum is the union of the components' spectra: every component is solved for
@@ -582,7 +585,12 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
     float64 solver), 'f32' (stored float32 blocks, no float64 phase) or 'mixed' (float32
     blocks for the bound, the filter design and the Lanczos phase, then float64 refinement of
     the Ritz block). ``v0``: optional start vector [3N] in the RCM order of the build, or a
-    callable A -> start vector (e.g. a learned warm start)."""
+    callable A -> start vector (e.g. a learned warm start). The filter's interval is sized by
+    KPM bisection to hold about ``kpm_factor`` x (modes + 6) eigenvalues per component, with
+    the designer's convergence ratio ``ratio`` (``filter_right``: the interval's right end
+    given directly instead); ``kpm_trials`` probe vectors (4 fill one
+    column group of the multi-vector product; on millions of dof the count's statistical
+    error stays far inside the bisection's margin)."""
     from . import chebyshev as CH
     ops.require_gpu()
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -595,18 +603,36 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
     C = len(seg_atoms) - 1
     t_build = time.time() - t0
     ks = int(per_component) if per_component else int(np.ceil(1.15 * k_total / C)) + 4
+    phases = {"build": t_build}
+    t = time.time()
     lo, hi = CH.spectrum_bound(A, precision=bulk, stop_rel=CH.BOUND_STOP_REL) if filtered else (None, None)
+    phases["bound"] = time.time() - t
+    t = time.time()
     start = v0(A, seg_atoms) if callable(v0) else v0
+    if start is not None:
+        phases["start_vector"] = time.time() - t
     for attempt in range(retries + 1):
         pol = None
         if filtered:
+            t = time.time()
             bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
-            pol, cnt, interval, _ = CH.optimize_on_memory(A, bound, A.plus_i, wanted_count=int(C * (ks + 6) * 1.5),
-                                                          designated_start=bound[1] / 10.0, max_degree=max_degree,
-                                                          min_degree=5, margin=max(5, C * ks // 10), trials=5,
-                                                          ratio=0.7, seed=seed, precision=bulk)
+            if filter_right is not None:
+                pol = CH.design_filter(A.plus_i, float(filter_right), bound, max_degree, 5, ratio)
+                cnt, interval = -1, (A.plus_i, float(filter_right))
+            else:
+                pol, cnt, interval, _ = CH.optimize_on_memory(A, bound, A.plus_i,
+                                                              wanted_count=int(C * (ks + 6) * kpm_factor),
+                                                              designated_start=bound[1] / 10.0,
+                                                              max_degree=max_degree, min_degree=5,
+                                                              margin=max(5, C * ks // 10), trials=kpm_trials,
+                                                              ratio=ratio, seed=seed, precision=bulk)
+            phases["filter"] = phases.get("filter", 0.0) + time.time() - t
+            filter_info = {"interval": [float(x) for x in interval], "kpm_count": int(cnt),
+                           "wanted_count": int(C * (ks + 6) * kpm_factor)}
+        t = time.time()
         r = solve_segments(A, seg_atoms, ks, precision=precision, tol_rel=tol_rel, maxiter=maxiter, seed=seed,
                            factor=factor, verbose=verbose, t0=t0, pol=pol, timer=timer, v0=start)
+        phases["solve"] = phases.get("solve", 0.0) + time.time() - t
         flat = r.eigval.reshape(-1)
         order = np.argsort(flat, kind="stable")[:k_total]
         cutoff = flat[order[-1]]
@@ -625,4 +651,5 @@ def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1
                        components=C, per_component=ks, restarts=r.restarts, n_spmv=r.n_spmv,
                        degree=int(pol.AdjustedDegree) if pol is not None else 0, A=A, seg_atoms=seg_atoms,
                        eigvec=r.eigvec, lam_max=r.lam_max, operator_launches=r.operator_launches,
-                       phase1_cycles=r.phase1_cycles, precision=precision)
+                       phase1_cycles=r.phase1_cycles, precision=precision, phases=phases,
+                       filter_info=filter_info if filtered else None)

@@ -79,19 +79,23 @@ __global__ void k_cell_count(int64_t n, const double *__restrict__ xyz, Grid g, 
   atomicAdd(&count[id], 1);
 }
 
-// exclusive scan of int32 counts into int32 starts (single block, chunked)
-__global__ void k_scan_i32(int64_t m, const int32_t *__restrict__ in, int32_t *__restrict__ out) {
-  __shared__ int32_t buf[1024];
-  __shared__ int32_t carry;
+// Exclusive scans (out[i] = sum in[0..i-1], out[m] = total). One block of 1024 threads
+// walks short arrays in chunks; longer ones take three passes: per-tile sums (kScanTile
+// elements per 256-thread block), the single-block scan of the tile sums, and a rescan of
+// every tile from its offset.
+template <typename T>
+__global__ void k_scan_one(int64_t m, const T *__restrict__ in, T *__restrict__ out) {
+  __shared__ T buf[1024];
+  __shared__ T carry;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   for (int64_t base = 0; base < m; base += 1024) {
-    int64_t i = base + threadIdx.x;
-    int32_t v = i < m ? in[i] : 0;
+    const int64_t i = base + threadIdx.x;
+    const T v = i < m ? in[i] : 0;
     buf[threadIdx.x] = v;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
-      int32_t t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      const T t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
       __syncthreads();
       buf[threadIdx.x] += t;
       __syncthreads();
@@ -102,6 +106,56 @@ __global__ void k_scan_i32(int64_t m, const int32_t *__restrict__ in, int32_t *_
     __syncthreads();
   }
   if (threadIdx.x == 0) out[m] = carry;
+}
+
+constexpr int kScanPer = 16;                 // elements per thread of a tile
+constexpr int kScanTile = 256 * kScanPer;    // elements per tile (one 256-thread block)
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scan_tile_sums(int64_t m, const T *__restrict__ in, T *__restrict__ sums) {
+  __shared__ T red[256];
+  const int64_t b0 = blockIdx.x * (int64_t)kScanTile;
+  T s = 0;
+  for (int q = threadIdx.x; q < kScanTile; q += 256) {
+    const int64_t i = b0 + q;
+    if (i < m) s += in[i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_scan_tiles(int64_t m, const T *__restrict__ in, const T *__restrict__ offs,
+                                                    T *__restrict__ out) {
+  __shared__ T ts[256];
+  const int64_t i0 = blockIdx.x * (int64_t)kScanTile + threadIdx.x * (int64_t)kScanPer;
+  T v[kScanPer];
+  T tot = 0;
+#pragma unroll
+  for (int q = 0; q < kScanPer; ++q) {
+    v[q] = i0 + q < m ? in[i0 + q] : 0;
+    tot += v[q];
+  }
+  ts[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const T t = (int)threadIdx.x >= off ? ts[threadIdx.x - off] : 0;
+    __syncthreads();
+    ts[threadIdx.x] += t;
+    __syncthreads();
+  }
+  T run = offs[blockIdx.x] + ts[threadIdx.x] - tot;
+#pragma unroll
+  for (int q = 0; q < kScanPer; ++q) {
+    if (i0 + q < m) out[i0 + q] = run;
+    run += v[q];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[m] = offs[gridDim.x];
 }
 
 __global__ void k_cell_fill(int64_t n, const int32_t *__restrict__ cell_of, const int32_t *__restrict__ start,
@@ -246,31 +300,6 @@ __global__ void k_image_masks(int64_t n, const double *__restrict__ xyz, double 
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) images[e] = (uint16_t)image_mask(xyz, i, indices[e], bx, by, rc2);
-}
-
-// exclusive scan of int64 counts -> indptr (single block)
-__global__ void k_scan_i64(int64_t m, const int64_t *__restrict__ in, int64_t *__restrict__ out) {
-  __shared__ int64_t buf[1024];
-  __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < m; base += 1024) {
-    int64_t i = base + threadIdx.x;
-    int64_t v = i < m ? in[i] : 0;
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      int64_t t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < m) out[i] = carry + buf[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += buf[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[m] = carry;
 }
 
 // One block (256 threads) per row: bitonic sort of the row's columns in LDS.
@@ -638,6 +667,39 @@ struct DevBuf {
     if (_e != hipSuccess) INCH_FAIL(INCH_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
   } while (0)
 
+namespace {
+
+// out[0..m] = exclusive scan of in[0..m-1] (out[m] = total), on stream st.
+template <typename T>
+inch_status scan_exclusive(int64_t m, const T *in, T *out, hipStream_t st) {
+  const int64_t nt = (m + kScanTile - 1) / kScanTile;
+  if (nt <= 4) {
+    k_scan_one<T><<<1, 1024, 0, st>>>(m, in, out);
+    HIP_TRY(hipGetLastError());
+    return INCH_OK;
+  }
+  DevBuf sums, offs;
+  HIP_TRY(hipMalloc(&sums.p, nt * sizeof(T)));
+  HIP_TRY(hipMalloc(&offs.p, (nt + 1) * sizeof(T)));
+  k_scan_tile_sums<T><<<(unsigned)nt, 256, 0, st>>>(m, in, (T *)sums.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_one<T><<<1, 1024, 0, st>>>(nt, (const T *)sums.p, (T *)offs.p);
+  HIP_TRY(hipGetLastError());
+  k_scan_tiles<T><<<(unsigned)nt, 256, 0, st>>>(m, in, (const T *)offs.p, out);
+  HIP_TRY(hipGetLastError());
+  // the temporaries are freed when this returns: wait for the kernels that read them
+  HIP_TRY(hipStreamSynchronize(st));
+  return INCH_OK;
+}
+
+}  // namespace
+
+#define SCAN_TRY(m, in, out)                                   \
+  do {                                                         \
+    const inch_status _s = scan_exclusive((m), (in), (out), st); \
+    if (_s != INCH_OK) return _s;                              \
+  } while (0)
+
 extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, double rc, int64_t *indptr,
                                                int32_t *indices, int64_t capacity, int64_t *nnz_out,
                                                void *stream) {
@@ -683,8 +745,7 @@ extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, dou
   const int64_t gb = (n + bs - 1) / bs;
   k_cell_count<<<gb, bs, 0, st>>>(n, xyz, g, (int32_t *)cell_of.p, (int32_t *)cnt.p);
   HIP_TRY(hipGetLastError());
-  k_scan_i32<<<1, 1024, 0, st>>>(ncell, (int32_t *)cnt.p, (int32_t *)start.p);
-  HIP_TRY(hipGetLastError());
+  SCAN_TRY(ncell, (const int32_t *)cnt.p, (int32_t *)start.p);
   HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
   k_cell_fill<<<gb, bs, 0, st>>>(n, (int32_t *)cell_of.p, (int32_t *)start.p, (int32_t *)cnt.p, (int32_t *)atoms.p);
   HIP_TRY(hipGetLastError());
@@ -694,8 +755,7 @@ extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, dou
     k_neigh<<<gb, bs, 0, st>>>(n, xyz, g, rc2, (int32_t *)start.p, (int32_t *)atoms.p, nullptr, (int64_t *)counts.p,
                                nullptr);
     HIP_TRY(hipGetLastError());
-    k_scan_i64<<<1, 1024, 0, st>>>(n, (int64_t *)counts.p, indptr);
-    HIP_TRY(hipGetLastError());
+    SCAN_TRY(n, (const int64_t *)counts.p, indptr);
     int64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -732,8 +792,7 @@ extern "C" inch_status inch_csr_permute(int64_t n, const int64_t *indptr, const 
   const int bs = 256;
   k_perm_counts<<<(unsigned)((n + bs - 1) / bs), bs, 0, st>>>(n, indptr, order, (int64_t *)counts.p);
   HIP_TRY(hipGetLastError());
-  k_scan_i64<<<1, 1024, 0, st>>>(n, (int64_t *)counts.p, out_indptr);
-  HIP_TRY(hipGetLastError());
+  SCAN_TRY(n, (const int64_t *)counts.p, out_indptr);
   k_permute_sort_rows<<<(unsigned)n, 256, 0, st>>>(n, indptr, indices, order, undo, out_indptr, out_indices,
                                                   (int32_t *)ovf.p);
   HIP_TRY(hipGetLastError());
@@ -809,8 +868,7 @@ extern "C" inch_status inch_neighbors_celllist_pbc(int64_t n, const double *xyz,
   const int64_t gb = (n + bs - 1) / bs;
   k_cell_count_pbc<<<gb, bs, 0, st>>>(n, xyz, g, (int32_t *)cell_of.p, (int32_t *)cnt.p);
   HIP_TRY(hipGetLastError());
-  k_scan_i32<<<1, 1024, 0, st>>>(ncell, (int32_t *)cnt.p, (int32_t *)start.p);
-  HIP_TRY(hipGetLastError());
+  SCAN_TRY(ncell, (const int32_t *)cnt.p, (int32_t *)start.p);
   HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
   k_cell_fill<<<gb, bs, 0, st>>>(n, (int32_t *)cell_of.p, (int32_t *)start.p, (int32_t *)cnt.p, (int32_t *)atoms.p);
   HIP_TRY(hipGetLastError());
@@ -820,8 +878,7 @@ extern "C" inch_status inch_neighbors_celllist_pbc(int64_t n, const double *xyz,
     k_neigh_pbc<<<gb, bs, 0, st>>>(n, xyz, g, rc2, (int32_t *)start.p, (int32_t *)atoms.p, nullptr,
                                    (int64_t *)counts.p, nullptr);
     HIP_TRY(hipGetLastError());
-    k_scan_i64<<<1, 1024, 0, st>>>(n, (int64_t *)counts.p, indptr);
-    HIP_TRY(hipGetLastError());
+    SCAN_TRY(n, (const int64_t *)counts.p, indptr);
     int64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -919,13 +976,14 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
     }
     if (nan || !std::isfinite(g.lo[a]) || !std::isfinite(hi[a])) INCH_FAIL(INCH_ERR_NAN, "inch_anm_clusters: non-finite coordinates");
   }
-  // cells: cell_xy in x and y; cell_z in z, coarsened until there are at most max(2n, 65536) cells
+  // cells: cell_xy in x and y; cell_z in z, coarsened only beyond max(16n, 65536) cells (an
+  // assembly's bounding box is mostly empty: coarser cells would make its clusters taller)
   const double size[3] = {cell_xy, cell_xy, cell_z};
   for (int a = 0; a < 3; ++a) {
     g.inv[a] = 1.0 / size[a];
     g.dim[a] = std::max<int64_t>(1, (int64_t)std::floor((hi[a] - g.lo[a]) * g.inv[a]) + 1);
   }
-  const int64_t cap_cells = std::max<int64_t>(2 * n, 65536);
+  const int64_t cap_cells = std::max<int64_t>(16 * n, 65536);
   while (g.dim[0] * g.dim[1] * g.dim[2] > cap_cells && g.dim[2] > 1) {
     g.inv[2] *= 0.5;
     g.dim[2] = std::max<int64_t>(1, (int64_t)std::floor((hi[2] - g.lo[2]) * g.inv[2]) + 1);
@@ -943,8 +1001,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   const int64_t gb = (n + bs - 1) / bs;
   k_cl_cell<<<gb, bs, 0, st>>>(n, xyz, g, (int32_t *)cell_of.p, (int32_t *)cnt.p);
   HIP_TRY(hipGetLastError());
-  k_scan_i32<<<1, 1024, 0, st>>>(ncell, (int32_t *)cnt.p, (int32_t *)start.p);
-  HIP_TRY(hipGetLastError());
+  SCAN_TRY(ncell, (const int32_t *)cnt.p, (int32_t *)start.p);
   HIP_TRY(hipMemsetAsync(cnt.p, 0, (ncell + 1) * sizeof(int32_t), st));
   k_cell_fill<<<gb, bs, 0, st>>>(n, (int32_t *)cell_of.p, (int32_t *)start.p, (int32_t *)cnt.p, (int32_t *)atoms.p);
   HIP_TRY(hipGetLastError());
@@ -981,8 +1038,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
                                                               (int64_t *)counts0.p, (int32_t *)base0.p, cluster_atoms,
                                                               (int64_t *)counts.p, base);
   HIP_TRY(hipGetLastError());
-  k_scan_i64<<<1, 1024, 0, st>>>(ncl, (int64_t *)counts.p, cl_ptr);
-  HIP_TRY(hipGetLastError());
+  SCAN_TRY(ncl, (const int64_t *)counts.p, cl_ptr);
   inch_status s = check_overflow("count");
   if (s != INCH_OK) return s;
   int64_t total = 0;

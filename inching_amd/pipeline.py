@@ -131,7 +131,7 @@ def lowest_modes(A, k, Q=None, precision="mixed", tol_rel=1e-8, filtered=True, b
             pol, cnt, interval, nk = CH.optimize_on_memory(
                 A, bound, A.plus_i, wanted_count=int(kpm_count) if kpm_count else max(3 * k, k + 32),
                 designated_start=bound[1] / 10.0,
-                max_degree=max_degree, min_degree=min_degree, margin=max(5, k // 2), trials=5, ratio=ratio,
+                max_degree=max_degree, min_degree=min_degree, margin=max(5, k // 2), trials=4, ratio=ratio,
                 seed=seed, precision="f32" if precision == "mixed" else precision)
             n_spmv["kpm"] = nk
         else:
