@@ -324,6 +324,99 @@ def optimize_on_memory(A, bound, wanted_left, wanted_count, designated_start=Non
     return pol, cnt, tuple(interval), kpm.n_spmv
 
 
+def jackson_kernel(M):
+    """Jackson damping factors g_0..g_M of a degree-M Chebyshev expansion (Weisse, Wellein,
+    Alvermann and Fehske, Rev. Mod. Phys. 78, 275 (2006), eq. 71): the damped expansion of a
+    non-negative function stays non-negative, with a resolution of about pi / M in the angle
+    arccos(x)."""
+    q = np.pi / (M + 1)
+    k = np.arange(M + 1, dtype=np.float64)
+    return ((M - k + 1) * np.cos(k * q) + np.sin(k * q) / np.tan(q)) / (M + 1)
+
+
+class KPMCounter:
+    """Number of eigenvalues of A in [bound[0], r] for any r, from the Chebyshev moments
+    mu_k = v^T T_k(A') v of ``probes`` random unit vectors (A' = (A - cc) / dd mapping the
+    bound onto [-1, 1]), with the Jackson damping of the expansion degree ``degree``:
+
+        count(r) = n (c_0 + sum_k g_k c_k mu_k),  c_0 = (pi - t) / pi,
+        c_k = -2 sin(k t) / (k pi),  t = arccos((r - cc) / dd).
+
+    Unlike the reference's estimate (which damps with the filter's kernel, ~1 for low
+    degrees, so narrow intervals at the end of a large spectrum give oscillating, even
+    negative, counts), this one is monotone in r and non-negative; its resolution near the
+    lower end of the spectrum is about dd (pi / degree)^2 / 2 in eigenvalue. Each moment is
+    one operator launch with the fused Chebyshev step plus one dot."""
+
+    def __init__(self, A, bound, degree=64, probes=1, seed=0, precision="mf64"):
+        from .autotune import best_config, bsr_launch
+        if precision != "mf64":
+            A.ensure(precision)
+        lo, hi = bound
+        self.cc, self.dd = 0.5 * (hi + lo), 0.5 * (hi - lo)
+        self.n = A.shape[0]
+        self.degree = int(degree)
+        dev = A.device
+        vdt = torch.float32 if precision == "f32" else torch.float64
+        cfg = best_config(A, precision)
+        blanes, c16 = bsr_launch(A, cfg) if precision != "mf64" else (0, None)
+        vals = A.vals(precision) if precision != "mf64" else None
+
+        def step(x, y, **epi):
+            if vals is None:
+                A.matfree_spmv(x, y, config=cfg, **epi)
+            else:
+                ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=blanes, col16=c16, **epi)
+
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(seed))
+        mom = torch.zeros((probes, self.degree + 1), dtype=torch.float64, device=dev)
+        ws = ops.Workspace(1, dev)
+        t = [torch.empty(self.n, dtype=vdt, device=dev) for _ in range(3)]
+        d64 = torch.empty(self.n, dtype=torch.float64, device=dev) if vdt != torch.float64 else None
+        for p in range(probes):
+            v = torch.randn(self.n, generator=g, dtype=torch.float64, device=dev)
+            v /= torch.linalg.norm(v)
+            vr = v.view(1, -1)
+            mom[p, 0] = 1.0
+            t[0].copy_(v)
+            prev, cur, nxt = None, t[0], t[1]
+            for k in range(1, self.degree + 1):
+                if k == 1:
+                    step(cur, nxt, s1=1.0 / self.dd, c0=self.cc)
+                else:
+                    step(cur, nxt, s1=2.0 / self.dd, c0=self.cc, z=prev, c2=1.0)
+                src = nxt
+                if d64 is not None:
+                    ops.convert(nxt, d64)
+                    src = d64
+                ops.multidot(vr, 1, src, mom[p, k:k + 1], ws)
+                free = prev if prev is not None else t[2]
+                prev, cur, nxt = cur, nxt, free
+        self.moments = mom.mean(0).cpu().numpy()
+        self.jackson = jackson_kernel(self.degree)
+        self.n_spmv = probes * self.degree
+
+    def count(self, r):
+        x = min(1.0, max(-1.0, (float(r) - self.cc) / self.dd))
+        t = np.arccos(x)
+        k = np.arange(1, self.degree + 1, dtype=np.float64)
+        ck = -2.0 * np.sin(k * t) / (k * np.pi)
+        return self.n * ((np.pi - t) / np.pi + float(np.sum(self.jackson[1:] * ck * self.moments[1:])))
+
+    def right_for(self, count, lo, hi, tol=1e-6):
+        """The r in [lo, hi] whose estimated count is ``count`` (bisection; the count is
+        monotone in r)."""
+        a, b = float(lo), float(hi)
+        while b - a > tol * max(1.0, abs(b)):
+            m = 0.5 * (a + b)
+            if self.count(m) < count:
+                a = m
+            else:
+                b = m
+        return 0.5 * (a + b)
+
+
 BOUND_STOP_REL = 1e-10   # early end of the bound run for callers that use only its maximum
 BOUND_CHECK_EVERY = 10
 

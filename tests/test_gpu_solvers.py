@@ -165,3 +165,26 @@ def test_device_kpm_bisection_interval_holds_wanted_count(problem, designated):
     assert abs(cnt - wanted) <= margin and nk > 0
     exact = int(np.sum((w >= 1.0 - 1e-9) & (w <= interval[1])))
     assert abs(exact - wanted) <= margin + 8, (exact, interval, cnt)
+
+
+@pytest.mark.parametrize("precision", ["mf64", "f32"])
+def test_kpm_counter_matches_exact_counts(problem, precision):
+    """The Jackson-damped KPM count of eigenvalues in [PlusI, r] against the exact count from
+    dense eigh (1,200 atoms, 3,600 dof; 16 probes, 64 moments), the bisection for r, and
+    monotonicity. Tolerance: 8 + 10 % of the exact count (smoothing of about 0.04 in eigenvalue
+    near the lower end plus the probes' statistical error)."""
+    from inching_amd import chebyshev as CH
+    A, Ad, w, U, Q = problem
+    print("case: KPM counter %s" % precision, flush=True)
+    bound = (1.0, w[-1] * (1 + 1e-3))
+    c = CH.KPMCounter(A, bound, degree=64, probes=16, seed=3, precision=precision)
+    prev = -1.0
+    for r in (2.0, 4.0, 8.0, 15.0, 30.0):
+        exact = int(np.sum(w <= r))
+        est = c.count(r)
+        assert abs(est - exact) <= 8 + 0.1 * exact, (r, est, exact)
+        assert est >= prev
+        prev = est
+    r = c.right_for(200.0, 1.0, bound[1])
+    assert abs(c.count(r) - 200.0) < 1.0
+    assert abs(np.sum(w <= r) - 200) <= 8 + 20
