@@ -46,7 +46,8 @@ def parse():
     ap.add_argument("--atoms", type=int, default=300000)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--mode", default="mf64", help="headline operator mode: mf64 | mixed | f64")
-    ap.add_argument("--compare", default="f64,mixed", help="other modes timed once each")
+    ap.add_argument("--compare", default="f64,mixed,warm",
+                    help="other modes timed once each (warm: the headline operator from the learned start)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
@@ -236,9 +237,18 @@ def run_assembly(args):
     xyz = torch.as_tensor(X, device=dev)
     K = args.modes
 
+    net = None
+
     def step(mode, timer=None):
-        return solve_assembly(xyz, K, precision=mode, tol_rel=args.tol, timer=timer, seed=0,
-                              verbose=args.verbose)
+        nonlocal net
+        v0 = None
+        if mode == "warm":                  # matrix-free float64 from the learned start vector
+            from inching_amd import warmstart as W
+            if net is None:
+                net = W.WarmStartNet.load()
+            v0 = lambda A_, seg: W.warm_start_vector(net, A_, seg)          # noqa: E731
+        return solve_assembly(xyz, K, precision="mf64" if mode == "warm" else mode, tol_rel=args.tol,
+                              timer=timer, seed=0, verbose=args.verbose, v0=v0)
 
     for w in range(args.warmup):           # also fills the autotuner's cache for this matrix
         step(args.mode)
@@ -271,7 +281,8 @@ def run_assembly(args):
                "filter_degree": r.degree, "lanczos_steps": int(r.n_spmv), "restarts": int(r.restarts),
                "operator_launches": int(r.operator_launches), "max_rel_res": float(np.max(r.rel_res)),
                "lam_1_minus_plus_i": float(r.eigval[0] - 1.0), "lam_k_minus_plus_i": float(r.eigval[-1] - 1.0),
-               "lam_max_bound": float(r.lam_max)}
+               "lam_max_bound": float(r.lam_max), "phases_s": {kk: round(v, 3) for kk, v in r.phases.items()},
+               "filter": r.filter_info}
     headline = r
     del A
 
@@ -287,7 +298,8 @@ def run_assembly(args):
         compare[mode] = {"seconds": round(time.time() - ta, 3), "max_rel_res": float(np.max(rr.rel_res)),
                          "lanczos_steps": int(rr.n_spmv), "restarts": int(rr.restarts),
                          "operator_launches": int(rr.operator_launches), "filter_degree": rr.degree,
-                         "max_abs_dlambda_vs_headline": float(np.max(np.abs(rr.eigval - headline.eigval)))}
+                         "max_abs_dlambda_vs_headline": float(np.max(np.abs(rr.eigval - headline.eigval))),
+                         "phases_s": {kk: round(v, 3) for kk, v in rr.phases.items()}}
         if rr.phase1_cycles is not None:
             compare[mode]["float32_phase_cycles"] = int(rr.phase1_cycles)
         print("bench: compare mode %s done in %.2f s" % (mode, compare[mode]["seconds"]), file=sys.stderr, flush=True)
@@ -302,9 +314,9 @@ def run_assembly(args):
             rf["frac_of_stream_copy"] = round(rf["achieved"] / copy_gbs, 4)
     if "f64" in compare:
         summary["speedup_vs_plain_f64"] = round(compare["f64"]["seconds"] / sec, 3)
-    if "mixed" in compare:
-        summary["speedup_mixed_vs_plain_f64"] = (round(compare["f64"]["seconds"] / compare["mixed"]["seconds"], 3)
-                                                 if "f64" in compare else None)
+    for mode in ("mixed", "warm"):
+        if mode in compare and "f64" in compare:
+            summary["speedup_%s_vs_plain_f64" % mode] = round(compare["f64"]["seconds"] / compare[mode]["seconds"], 3)
 
     cpu = None
     if not args.no_cpu:
@@ -658,7 +670,8 @@ def run_globule(args):
                    "phases_s": {k: round(v, 4) for k, v in r.phases.items()},
                    "max_rel_res": float(np.max(r.rel_res)), "lam_min": float(r.eigval[0]),
                    "speedup_vs_plain_f64": (round(compare["f64"]["seconds"] / sec, 3) if "f64" in compare else None),
-                   "lam_max_bound": float(r.lam_max)},
+                   "lam_max_bound": float(r.lam_max), "phases_s": {kk: round(v, 3) for kk, v in r.phases.items()},
+               "filter": r.filter_info},
         "roofline": roof,
         "roofline_block_spmv_f32": roof_block,
         "kernels": kernel_table,

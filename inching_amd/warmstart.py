@@ -41,15 +41,30 @@ P_ = ops.P
 SP = ops.stream_ptr
 
 
-def features(xyz, rowptr):
-    """[N, 16] float32 input features of centred coordinates (any length unit)."""
+def features(xyz, rowptr, seg_atoms=None):
+    """[N, 16] float32 input features of centred coordinates (any length unit). With
+    ``seg_atoms`` (offsets [S+1] of contiguous atom ranges, e.g. the components of an
+    assembly) every range is centred, scaled and degree-normalised on its own, as if it were
+    a structure by itself."""
     x = xyz.to(torch.float64)
-    c = x - x.mean(0, keepdim=True)
-    R = torch.sqrt((c * c).sum(1).max()).clamp_min(1e-12)
+    deg = (rowptr[1:] - rowptr[:-1]).to(torch.float32)[:, None]
+    if seg_atoms is None:
+        c = x - x.mean(0, keepdim=True)
+        R = torch.sqrt((c * c).sum(1).max()).clamp_min(1e-12)
+        deg = deg / deg.mean()
+    else:
+        sizes = torch.as_tensor(np.diff(np.asarray(seg_atoms, dtype=np.int64)), device=x.device)
+        seg = torch.repeat_interleave(torch.arange(sizes.numel(), device=x.device), sizes)
+        cnt = sizes.to(torch.float64)[:, None]
+        mean = torch.zeros((sizes.numel(), 3), dtype=torch.float64, device=x.device).index_add_(0, seg, x) / cnt
+        c = x - mean[seg]
+        r2 = (c * c).sum(1)
+        R = torch.zeros(sizes.numel(), dtype=torch.float64, device=x.device).scatter_reduce_(
+            0, seg, r2, reduce="amax", include_self=False).sqrt().clamp_min(1e-12)[seg][:, None]
+        dm = torch.zeros((sizes.numel(), 1), dtype=torch.float32, device=x.device).index_add_(0, seg, deg)
+        deg = deg / (dm / cnt.to(torch.float32))[seg]
     u = (c / R).to(torch.float32)
     r = torch.sqrt((u * u).sum(1, keepdim=True))
-    deg = (rowptr[1:] - rowptr[:-1]).to(torch.float32)[:, None]
-    deg = deg / deg.mean()
     q = torch.stack([u[:, 0] * u[:, 0], u[:, 1] * u[:, 1], u[:, 2] * u[:, 2],
                      u[:, 0] * u[:, 1], u[:, 1] * u[:, 2], u[:, 0] * u[:, 2]], 1)
     f = torch.cat([u, r, q, deg, torch.ones_like(r)], 1)          # 12
@@ -166,10 +181,12 @@ class WarmStartNet:
         bo = torch.zeros(3 * self.C, device=dev).requires_grad_()
         self.params += [Wo, bo]
 
-    def forward(self, xyz, rowptr, col):
-        """Block P [3N, C] (C <= 32: the head emits 3C <= 96 channels).(float32) for coordinates in the matrix's atom order."""
+    def forward(self, xyz, rowptr, col, seg_atoms=None):
+        """Block P [3N, C] (float32; C <= 32: the head emits 3C channels) for coordinates in
+        the matrix's atom order; ``seg_atoms``: disconnected parts featurised one by one (the
+        graph layers never cross between them)."""
         gph = _Graph(rowptr, col)
-        h = features(xyz, rowptr)
+        h = features(xyz, rowptr, seg_atoms)
         for l in range(LAYERS):
             Ws, Wn, b = self.params[3 * l:3 * l + 3]
             h = GraphLayer.apply(h, Ws, Wn, b, gph)
@@ -237,3 +254,15 @@ def warm_block(net, A, Q=None, lam_max=None):
     rows = P.T.to(torch.float64).contiguous()
     theta, X, info = polish(A, rows, Q=Q, lam_max=lam_max)
     return theta, X, info
+
+
+def warm_start_vector(net, A, seg_atoms=None, seed=0):
+    """A start vector [3N] (float64) for a lockstep solve: a combination of the predicted
+    block's vector fields with positive random weights, featurised per component."""
+    with torch.no_grad():
+        P = net.forward(A.xyz, A.rowptr, A.col, seg_atoms)
+    g = torch.Generator(device=P.device)
+    g.manual_seed(int(seed) + 7)
+    w = torch.rand(P.shape[1], generator=g, dtype=torch.float32, device=P.device) + 0.5
+    return (P @ w).to(torch.float64)
+
