@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--modes", type=int, default=1000, help="modes of the assembly workload")
     ap.add_argument("--spmv-only", type=int, default=0, metavar="REPS",
                     help="only build the matrix and launch each operator kernel REPS times (profiling runs)")
+    ap.add_argument("--dense-only", type=int, default=0, metavar="REPS",
+                    help="only launch the float64 MFMA kernels (Gram, Rayleigh-Ritz combinations) REPS times at "
+                         "the solvers' shapes (profiling runs)")
     return ap.parse_args()
 
 
@@ -114,6 +117,49 @@ def spmv_only(args):
     out["stream_copy_bytes_read"] = src.numel() * 8
     from inching_amd.autotune import table
     out["autotune_ms"] = {k: v["ms"] for k, v in table().items()}
+    print(json.dumps(out))
+
+
+def dense_only(args):
+    """Short program for counter collection: the MFMA Gram and Rayleigh-Ritz kernels at the
+    shapes of the solves -- the 300k-atom globule's restart (basis of 192 rows, 64 Ritz
+    vectors, 900k dof) and the 2.4M-atom assembly's lockstep restart (84 rows, 24 vectors per
+    component, 60 components of 120k dof) -- REPS launches each, with their mean times."""
+    import numpy as np
+    import torch
+
+    from inching_amd import ops
+    from inching_amd.batch import SegOps, Segments
+    dev = torch.device("cuda", 0)
+    out = {}
+
+    def timed(name, fn, by, fl):
+        for _ in range(2):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.dense_only):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.dense_only
+        out[name] = {"ms": round(ms, 4), "GBs": round(by / ms / 1e6, 1), "TFLOPs": round(fl / ms / 1e9, 2)}
+
+    m, k, n = 192, 64, 900000
+    V = torch.randn((m, n), dtype=torch.float64, device=dev)
+    W = torch.randn((m, k), dtype=torch.float64, device=dev)
+    Y = torch.empty((k, n), dtype=torch.float64, device=dev)
+    ws = ops.GramWorkspace(k, k, dev)
+    timed("rr_combine m192 k64 n900k", lambda: ops.rr_combine(V, m, W, Y), (m + k) * n * 8, 2.0 * m * k * n)
+    timed("gram k64 x k64 n900k", lambda: ops.gram(Y, Y, ws=ws), 2 * k * n * 8, 2.0 * k * k * n)
+    del V, W, Y
+    m, k, S, per = 84, 24, 60, 120000
+    seg = Segments(np.arange(S + 1, dtype=np.int64) * per, dev)
+    so = SegOps(seg, m + 1, dev)
+    V = torch.randn((m, S * per), dtype=torch.float64, device=dev)
+    W = torch.randn((S, m, k), dtype=torch.float64, device=dev)
+    Y = torch.empty((k, S * per), dtype=torch.float64, device=dev)
+    timed("seg_combine m84 k24 S60 n7.2M", lambda: so.combine(V, m, W, Y), (m + k) * S * per * 8, 2.0 * m * k * S * per)
     print(json.dumps(out))
 
 
@@ -689,6 +735,8 @@ def main():
         sys.exit(2)
     if args.spmv_only:
         return spmv_only(args)
+    if args.dense_only:
+        return dense_only(args)
     return {"assembly": run_assembly, "globule": run_globule, "batch": run_batch, "warm": run_warm,
             "cpu_plain": run_cpu_plain}[args.workload](args)
 

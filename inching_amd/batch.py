@@ -233,7 +233,7 @@ def _projected_ritz(a, b, bk, k, filtered):
 def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
                    factor=2, seed=0, use_graph=True, verbose=False, t0=None, pol=None, dcgs2=True,
                    segment=None, early_stop=True, phase1_rel=3e-7, v0=None, timer=None, band=None,
-                   deflate="shift"):
+                   deflate=None):
     """Lockstep thick-restart Lanczos over the contiguous atom segments of a block-diagonal
     A: k modes per segment, per-segment rigid-body Hotelling deflation, optional common
     Chebyshev filter ``pol`` (largest k of p(A) per segment, then Rayleigh quotients).
@@ -258,7 +258,8 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     deflation); 'project' applies (I - Q Q^T) after the operator instead, so the rigid-body
     directions become zero eigenvalues of the operator rather than outliers at -shift (a
     filtered operator's rigid modes otherwise sit far below its spectrum, which costs Lanczos
-    steps).
+    steps). None (default): 'project' with a filter (whose wanted end is the top, far from
+    zero), 'shift' without one (the wanted end is the bottom, where zero would be).
 
     ``band`` = (p_r, r) for a filtered operator: only the pairs inside the filter's band
     [PlusI, r] have to converge -- during a cycle those whose Ritz value of p(A) is at least
@@ -305,6 +306,8 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     ah = torch.empty((ncv, S), dtype=torch.float64, pin_memory=True)
     bh = torch.empty((ncv, S), dtype=torch.float64, pin_memory=True)
 
+    if deflate is None:
+        deflate = "project" if filtered else "shift"
     project = deflate == "project"
     if deflate not in ("shift", "project"):
         raise ValueError("deflate must be 'shift' or 'project'")
@@ -603,7 +606,7 @@ def _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, maxiter
 def solve_assembly(xyz, k_total, rc=8.0, plus_i=1.0, precision="mf64", tol_rel=1e-8, per_component=None,
                    filtered=True, max_degree=5120, maxiter=300, seed=0, verbose=False, factor=3.5, retries=2,
                    timer=None, v0=None, band_factor=1.3, kpm_degree=64, kpm_probes=1, ratio=0.7, filter_right=None,
-                   filter_stretch=2.6, deflate="project"):
+                   filter_stretch=2.6, deflate=None):
     """Lowest ``k_total`` non-trivial modes of a structure made of disconnected components
     (e.g. subunits of an assembly farther apart than the cutoff). H is block-diagonal, so its
     spectrum is the union of the components' spectra: every component is solved for

@@ -264,30 +264,66 @@ __global__ void __launch_bounds__(256) k_seg_scale(const int64_t *__restrict__ c
 }
 
 // out[q*ld_out + i] = sum_r W[s][r][q] * V[r*ld + i]  (W: [S, m, k] row-major), q < k <= KM
-template <int KM>
-__global__ void __launch_bounds__(256) k_seg_combine(const int64_t *__restrict__ cbeg, const int64_t *__restrict__ cend,
-                                                      const int32_t *__restrict__ cseg, int m, int k,
-                                                      const double *__restrict__ V, int64_t ld,
-                                                      const double *__restrict__ W, double *__restrict__ out,
-                                                      int64_t ld_out) {
-  extern __shared__ double ws[];  // [m][k]
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// out[q, i] = sum_r W[s][r][q] V[r, i] over chunk c (segment s), q < k <= 16 NT, with
+// v_mfma_f64_16x16x4f64 (operand layouts as in k_rr_combine_mfma): W[s] is staged in LDS as
+// [mp][16 NT + 1] (mp = m rounded up to 4, zero rows and columns beyond m and k); every wave
+// takes 4 16-column slices of the chunk per pass (4 loads of V in flight) and keeps NT
+// accumulator tiles of 16 x 16 for each.
+template <int NT>
+__global__ void __launch_bounds__(256) k_seg_combine_mfma(const int64_t *__restrict__ cbeg,
+                                                           const int64_t *__restrict__ cend,
+                                                           const int32_t *__restrict__ cseg, int m, int k,
+                                                           const double *__restrict__ V, int64_t ld,
+                                                           const double *__restrict__ W, double *__restrict__ out,
+                                                           int64_t ld_out) {
+  extern __shared__ double ws[];  // [mp][16 NT + 1]
+  constexpr int KP = 16 * NT + 1;
   const int c = blockIdx.x;
   const int s = cseg[c];
-  for (int t = threadIdx.x; t < m * k; t += blockDim.x) ws[t] = W[(int64_t)s * m * k + t];
+  const int mp = (m + 3) & ~3;
+  for (int t = threadIdx.x; t < mp * KP; t += blockDim.x) {
+    const int r = t / KP, q = t % KP;
+    ws[t] = (r < m && q < k) ? W[((int64_t)s * m + r) * k + q] : 0.0;
+  }
   __syncthreads();
-  for (int64_t i = cbeg[c] + threadIdx.x; i < cend[c]; i += blockDim.x) {
-    double a[KM];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t e = cend[c];
+  constexpr int NS = 4;  // 16-column slices per wave and pass (loads in flight)
+  for (int64_t i0 = cbeg[c] + 16 * w; i0 < e; i0 += 64 * NS) {
+    int64_t col[NS];
+    double4_t acc[NS][NT];
 #pragma unroll
-    for (int q = 0; q < KM; ++q) a[q] = 0.0;
-    for (int r = 0; r < m; ++r) {
-      const double v = V[(int64_t)r * ld + i];
+    for (int s2 = 0; s2 < NS; ++s2) {
+      col[s2] = i0 + 64 * s2 + (lane & 15);
 #pragma unroll
-      for (int q = 0; q < KM; ++q)
-        if (q < k) a[q] += ws[r * k + q] * v;
+      for (int t = 0; t < NT; ++t) acc[s2][t] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    }
+    for (int r0 = 0; r0 < mp; r0 += 4) {
+      const int r = r0 + (lane >> 4);
+      double bv[NS];
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) bv[s2] = (col[s2] < e && r < m) ? V[(int64_t)r * ld + col[s2]] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double av = ws[r * KP + 16 * t + (lane & 15)];
+#pragma unroll
+        for (int s2 = 0; s2 < NS; ++s2)
+          acc[s2][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[s2], acc[s2][t], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int q = 0; q < KM; ++q)
-      if (q < k) out[(int64_t)q * ld_out + i] = a[q];
+    for (int s2 = 0; s2 < NS; ++s2) {
+      if (col[s2] >= e) continue;
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int q = 16 * t + (lane >> 4) + 4 * x;
+          if (q < k) out[(int64_t)q * ld_out + col[s2]] = acc[s2][t][x];
+        }
+    }
   }
 }
 
@@ -415,16 +451,16 @@ inch_status inch_seg_combine(int32_t nchunk, const int64_t *cbeg, const int64_t 
                              void *stream) {
   INCH_REQUIRE(nchunk > 0 && cbeg && cend && cseg && V && W && out && m > 0 && k > 0 && k <= 64,
                "inch_seg_combine: bad args (k <= 64)");
-  INCH_REQUIRE((int64_t)m * k * 8 <= 64 * 1024, "inch_seg_combine: m*k too large for LDS");
+  const int nt = k <= 16 ? 1 : (k <= 32 ? 2 : 4);
+  const size_t lds = (size_t)((m + 3) & ~3) * (16 * nt + 1) * sizeof(double);
+  INCH_REQUIRE(lds <= 64 * 1024, "inch_seg_combine: W[s] (m = %d, k = %d) does not fit the 64 KiB of LDS", m, k);
   hipStream_t st = (hipStream_t)stream;
-  if (k <= 8)
-    k_seg_combine<8><<<nchunk, 256, (size_t)m * k * sizeof(double), st>>>(cbeg, cend, cseg, m, k, V, ld, W, out, ld_out);
-  else if (k <= 32)
-    k_seg_combine<32><<<nchunk, 256, (size_t)m * k * sizeof(double), st>>>(cbeg, cend, cseg, m, k, V, ld, W, out,
-                                                                           ld_out);
+  if (nt == 1)
+    k_seg_combine_mfma<1><<<nchunk, 256, lds, st>>>(cbeg, cend, cseg, m, k, V, ld, W, out, ld_out);
+  else if (nt == 2)
+    k_seg_combine_mfma<2><<<nchunk, 256, lds, st>>>(cbeg, cend, cseg, m, k, V, ld, W, out, ld_out);
   else
-    k_seg_combine<64><<<nchunk, 256, (size_t)m * k * sizeof(double), st>>>(cbeg, cend, cseg, m, k, V, ld, W, out,
-                                                                           ld_out);
+    k_seg_combine_mfma<4><<<nchunk, 256, lds, st>>>(cbeg, cend, cseg, m, k, V, ld, W, out, ld_out);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

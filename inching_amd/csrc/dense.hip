@@ -19,7 +19,8 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 64;    // output tile 64 x 64 per workgroup (4 waves x 16 rows)
 constexpr int kChunk = 32;   // columns of n staged in LDS per step
-constexpr int kGramBlocks = 256;
+constexpr int kGramBlocks = 256;   // fixed grid of the residual-norm reduction
+constexpr int kGramGrid = 1024;    // most column blocks of a Gram product (at least 1,024 columns each)
 
 // One workgroup = 4 waves; wave w owns output rows [16w, 16w+16) of the tile and the 4
 // column blocks of 16. Operands are staged through LDS with coalesced row loads; the
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(256) k_resnorm(int64_t len, const double *__re
 
 extern "C" {
 
-int64_t inch_gram_work_size(int32_t k1, int32_t k2) { return (int64_t)kGramBlocks * k1 * k2; }
+int64_t inch_gram_work_size(int32_t k1, int32_t k2) { return (int64_t)kGramGrid * k1 * k2; }
 
 inch_status inch_gram(int64_t len, int32_t k1, const double *X, int64_t ldx, int32_t k2, const double *Y,
                       int64_t ldy, double *G, double *work, void *stream) {
@@ -132,8 +133,10 @@ inch_status inch_gram(int64_t len, int32_t k1, const double *X, int64_t ldx, int
                "inch_gram: bad args");
   INCH_REQUIRE(k1 <= 4096 && k2 <= 4096, "inch_gram: k too large");
   hipStream_t st = (hipStream_t)stream;
-  int64_t gx = (len + 4095) / 4096;
-  if (gx > kGramBlocks) gx = kGramBlocks;
+  // enough column blocks to fill the chip when k1 and k2 give one output tile (900k columns:
+  // 879 blocks instead of 220 with 4,096 columns each, 0.72 -> see the dense-kernel profile)
+  int64_t gx = (len + 1023) / 1024;
+  if (gx > kGramGrid) gx = kGramGrid;
   if (gx < 1) gx = 1;
   dim3 grid((unsigned)gx, (k1 + kTile - 1) / kTile, (k2 + kTile - 1) / kTile);
   k_gram_mfma<<<grid, 256, 0, st>>>(len, k1, X, ldx, k2, Y, ldy, work);

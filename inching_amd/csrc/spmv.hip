@@ -699,37 +699,62 @@ __global__ void k_convert(int64_t len, const S *__restrict__ s, D *__restrict__ 
     d[i] = (D)s[i];
 }
 
-template <int QT>
-__global__ void __launch_bounds__(256) k_rr_combine(int64_t len, int m, int k, const double *__restrict__ V, int64_t ld,
-                                                     const double *__restrict__ W, double *__restrict__ out,
-                                                     int64_t ld_out) {
-  // one thread per column i; W streamed through LDS in chunks of 64 rows
-  __shared__ double ws[64 * QT];
-  const int q0 = blockIdx.y * QT;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  double a[QT];
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+// out[q, i] = sum_r W[r, q] V[r, i] (q < k, r < m) with v_mfma_f64_16x16x4f64. A workgroup
+// owns 16 NT outputs q0.. (NT <= 4 tiles of 16) and 64 NS columns; wave w takes the 16-column
+// slices w, w + 4, ... of them, NS slices, each with NT accumulator tiles that share the
+// slice's V operand. W goes through LDS in chunks of 64 rows, staged once per workgroup for all its
+// NS x 64 columns. Operand layouts of the instruction: A (16 x 4): lane l holds
+// A[l % 16][l / 16]; B (4 x 16): lane l holds B[l / 16][l % 16]; D (16 x 16): lane l, element
+// e holds D[l / 16 + 4 e][l % 16]. Here A[q][r] = W[r][q] and B[r][i] = V[r][i]: each lane
+// loads one element of V per slice and 4 rows (16 consecutive columns of 4 rows per wave).
+template <int NS, int NT>
+__global__ void __launch_bounds__(256) k_rr_combine_mfma(int64_t len, int m, int k, const double *__restrict__ V,
+                                                          int64_t ld, const double *__restrict__ W,
+                                                          double *__restrict__ out, int64_t ld_out) {
+  __shared__ double ws[64][65];
+  const int q0 = blockIdx.y * 16 * NT;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  int64_t col[NS];
 #pragma unroll
-  for (int q = 0; q < QT; ++q) a[q] = 0.0;
+  for (int s = 0; s < NS; ++s) col[s] = (int64_t)blockIdx.x * (64 * NS) + 64 * s + 16 * w + (lane & 15);
+  double4_t acc[NS][NT];
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[s][t] = (double4_t){0.0, 0.0, 0.0, 0.0};
   for (int rc = 0; rc < m; rc += 64) {
     const int nr = (m - rc) < 64 ? (m - rc) : 64;
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * QT; t += blockDim.x) {
-      const int r = t / QT, q = t % QT;
-      ws[t] = (r < nr && q0 + q < k) ? W[(int64_t)(rc + r) * k + q0 + q] : 0.0;
+    for (int t = tid; t < 64 * 16 * NT; t += 256) {
+      const int r = t / (16 * NT), q = t % (16 * NT);
+      ws[r][q] = (r < nr && q0 + q < k) ? W[(int64_t)(rc + r) * k + q0 + q] : 0.0;
     }
     __syncthreads();
-    if (i < len) {
-      for (int r = 0; r < nr; ++r) {
-        const double v = V[(int64_t)(rc + r) * ld + i];
+    for (int r0 = 0; r0 < nr; r0 += 4) {
+      const int r = r0 + (lane >> 4);
+      double bv[NS];
 #pragma unroll
-        for (int q = 0; q < QT; ++q) a[q] += ws[r * QT + q] * v;
+      for (int s = 0; s < NS; ++s) bv[s] = (col[s] < len && r < nr) ? V[(int64_t)(rc + r) * ld + col[s]] : 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const double av = ws[r0 + (lane >> 4)][16 * t + (lane & 15)];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) acc[s][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[s], acc[s][t], 0, 0, 0);
       }
     }
   }
-  if (i < len) {
 #pragma unroll
-    for (int q = 0; q < QT; ++q)
-      if (q0 + q < k) out[(int64_t)(q0 + q) * ld_out + i] = a[q];
+  for (int s = 0; s < NS; ++s) {
+    if (col[s] >= len) continue;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int q = q0 + 16 * t + (lane >> 4) + 4 * e;
+        if (q < k) out[(int64_t)q * ld_out + col[s]] = acc[s][t][e];
+      }
   }
 }
 
@@ -1448,16 +1473,18 @@ inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, 
                             double *out, int64_t ld_out, void *stream) {
   INCH_REQUIRE(len > 0 && m > 0 && k > 0 && V && W && out && ld >= len && ld_out >= len, "inch_rr_combine: bad args");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t gx = (len + 255) / 256;
-  INCH_REQUIRE(gx < ((int64_t)1 << 31), "inch_rr_combine: len too large");
-  if (k > 16) {
-    constexpr int QT = 32;
-    dim3 grid((unsigned)gx, (k + QT - 1) / QT);
-    k_rr_combine<QT><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
+  // 4 output tiles with 4 slices (256 columns) per workgroup, fewer tiles with 8 slices (measured
+  // best: 8 slices with 4 tiles took 0.74 against 0.68 ms for m = 192, k = 64, 900k columns)
+  INCH_REQUIRE(len < ((int64_t)1 << 38) && k <= 64 * 65535, "inch_rr_combine: len or k too large");
+  if (k > 32) {
+    dim3 grid((unsigned)((len + 255) / 256), (unsigned)((k + 63) / 64));
+    k_rr_combine_mfma<4, 4><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
+  } else if (k > 16) {
+    dim3 grid((unsigned)((len + 511) / 512), 1);
+    k_rr_combine_mfma<8, 2><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
   } else {
-    constexpr int QT = 16;
-    dim3 grid((unsigned)gx, (k + QT - 1) / QT);
-    k_rr_combine<QT><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
+    dim3 grid((unsigned)((len + 511) / 512), 1);
+    k_rr_combine_mfma<8, 1><<<grid, 256, 0, st>>>(len, m, k, V, ld, W, out, ld_out);
   }
   HIP_TRY(hipGetLastError());
   return INCH_OK;

@@ -305,7 +305,7 @@ inch_status inch_dcgs2_step_noalloc(int64_t len, int32_t J, double *V, int64_t l
                                     const double *beta, double *beta_out, double *vnext, void *stream);
 
 /* out[q*ld_out + i] = sum_r W[r*k + q] * V[r*ld + i]  for q < k, r < m
- * (W float64 [m, k] row-major on device; out must not alias V). */
+ * (W float64 [m, k] row-major on device; out must not alias V). FP64 MFMA 16x16x4. */
 inch_status inch_rr_combine(int64_t len, int32_t m, int32_t k, const double *V, int64_t ld,
                             const double *W, double *out, int64_t ld_out, void *stream);
 
@@ -361,7 +361,8 @@ inch_status inch_seg_dcgs2_step(int32_t nchunk, int32_t S, const int64_t *cbeg, 
 /* dst[i] = src[i] / div[s] (and dst2). */
 inch_status inch_seg_scale(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
                            const double *src, const double *div, double *dst, double *dst2, void *stream);
-/* out[q*ld_out+i] = sum_r W[s][r][q] V[r*ld+i] (W [S, m, k], k <= 64, m*k <= 8192). */
+/* out[q*ld_out+i] = sum_r W[s][r][q] V[r*ld+i] (W [S, m, k], k <= 64; W[s] padded to
+ * [4 ceil(m/4)][16 ceil(k/16) + 1] doubles must fit 64 KiB of LDS). FP64 MFMA 16x16x4. */
 inch_status inch_seg_combine(int32_t nchunk, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg, int32_t m,
                              int32_t k, const double *V, int64_t ld, const double *W, double *out, int64_t ld_out,
                              void *stream);

@@ -24,8 +24,11 @@ rcm.rcm_order_levels          level-synchronous form (the device algorithm):
                               directions.
 partition.batch_head          closed form of recursive halving (leaf sizes in
                               (L/2, L], sum = N) on many N.
-partition.ungapped_islands    brute-force definition check (every neighbour
-                              column of a batch lies in an island; gap rule).
+partition.ungapped_islands    hand-derived islands of a planted CSR whose
+                              early-index, fifth-last, diagonal and width
+                              exceptions of the gap rule each change the
+                              answer; brute-force coverage (every neighbour
+                              column of a batch lies in an island).
 hessian.*                     closed form -d d^T/|d|^2; invariants (symmetry,
                               translations and infinitesimal rotations in the
                               null space of H, tr H = 2 #contacts, PSD);
@@ -43,7 +46,15 @@ chebyshev.optimize_on_degree  golden vectors written by importing the
                               response p(x) = sum mu_k T_k(x) vs numpy chebval.
 chebyshev.cheb_av             U p(L) U^T x by dense eigh on small H.
 chebyshev.kpm_estimate        statistical only (mean count vs dense eigh count).
+chebyshev.optimize_on_memory  closed-form spectrum: the chosen interval holds the
+                              wanted count within the margin (both
+                              designated-start branches).
 lanczos.spectrum_bound        lambda_max estimate <= true lambda_max and close.
+analysis.sqfluct              with every non-rigid eigenpair of a small H,
+                              sum_m RatioSqFPM^2 = lambda_min tr(H^+)_aa.
+analysis.cross_correlation    self-pair maps undone and summed over every
+                              eigenpair = 3x3 block traces of H^2; hand-built
+                              single-atom modes fix the orientation and sign.
 lanczos.trlm / jdm / irlm     final eigenpairs vs dense numpy.linalg.eigh
                               (eigenvalues, subspace angles, residuals).
                               Iterates and iteration counts are parity

@@ -53,6 +53,13 @@ def test_batched_segmented_kernels(cuda):
     for s in range(4):
         sl = slice(seg_ptr[s], seg_ptr[s + 1])
         assert np.allclose(out.cpu().numpy()[:, sl], W[s].T @ V[:, sl], rtol=1e-11, atol=1e-10)
+    for kk in (17, 40, 64):            # two and four 16-wide output tiles
+        W = rng.standard_normal((4, J, kk))
+        out = torch.empty((kk, n), dtype=torch.float64, device="cuda")
+        so.combine(Vd, J, torch.as_tensor(W, device="cuda"), out)
+        for s in range(4):
+            sl = slice(seg_ptr[s], seg_ptr[s + 1])
+            assert np.allclose(out.cpu().numpy()[:, sl], W[s].T @ V[:, sl], rtol=1e-11, atol=1e-10), (kk, s)
 
 
 @pytest.mark.parametrize("J", [11, 70, 150])

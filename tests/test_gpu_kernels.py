@@ -259,6 +259,12 @@ def test_dense_kernels(cuda):
     out2 = torch.empty((5, n), dtype=torch.float64, device="cuda")
     ops.rr_combine(Vd, J, _dev(W2), out2)
     assert np.allclose(out2.cpu().numpy(), W2.T @ V[:J], rtol=1e-11, atol=1e-9)
+    # more than 64 rows (two LDS chunks of W) and more than 64 outputs (two output blocks)
+    Vb = rng.standard_normal((150, 20011))
+    W3 = rng.standard_normal((150, 90))
+    out3 = torch.empty((90, 20011), dtype=torch.float64, device="cuda")
+    ops.rr_combine(_dev(Vb), 150, _dev(W3), out3)
+    assert np.allclose(out3.cpu().numpy(), W3.T @ Vb, rtol=1e-11, atol=1e-9)
     dst, dst2 = torch.empty(n, dtype=torch.float64, device="cuda"), torch.empty(n, dtype=torch.float64, device="cuda")
     ops.scale_div(ud, nrm, dst, dst2)
     assert torch.allclose(dst, ud / nrm) and torch.equal(dst, dst2)
