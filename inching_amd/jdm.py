@@ -54,6 +54,7 @@ class JDM:
         self.mx = torch.zeros(self.n, **f64)
         self.h = torch.zeros(max(self.jmax + 1, self.restart + 1, self.jmin + 1), **f64)
         self.s1 = torch.zeros(1, **f64)
+        self.Hd = torch.zeros((self.restart, self.restart + 1), **f64)   # row j = column j of H
         self.ws = ops.Workspace(self.h.numel(), dev)
 
     # -- small device helpers ------------------------------------------------
@@ -76,8 +77,18 @@ class JDM:
         ops.axpby(-sigma, x, 1.0, out)
         return out
 
+    def _project(self, uu, x):
+        """x -= (uu . x) uu with the dot kept on the device."""
+        ops.multidot(uu.view(1, -1), 1, x, self.s1, self.ws)
+        ops.axpy_dev(self.s1, -1.0, uu, x)
+
     def _gmres(self, b, uu, sigma, tol, maxiter, rng):
-        """gmresDeflate: returns the projected iterate (the reference returns mx)."""
+        """gmresDeflate: returns the projected iterate (the reference returns mx).
+
+        The Arnoldi cycle stays on the device: projections use device dots, the Hessenberg
+        column and the next basis vector come from the dot and norm kernels without a host
+        round trip; the (R+1) x R Hessenberg matrix is read once per cycle for the
+        least-squares solve, and the residual norm once per restart."""
         n, R = self.n, self.restart
         x, mx = self.x, self.mx
         x.copy_(torch.as_tensor(rng.random(n) * (tol / n), device=x.device))
@@ -86,40 +97,44 @@ class JDM:
             mx.copy_(b)
             return mx
         atol = tol * b_norm
+        Hd = self.Hd
         iters = 0
         while True:
             mx.copy_(x)
-            c = self._dot(uu, mx)
-            ops.axpby(-c, uu, 1.0, mx)
+            self._project(uu, mx)
             self.op.apply(mx, self.w)
             ops.axpby(-sigma, mx, 1.0, self.w)
-            c = self._dot(uu, self.w)
-            ops.axpby(-c, uu, 1.0, self.w)
+            self._project(uu, self.w)
             # w <- b - w
             ops.axpby(1.0, b, -1.0, self.w)
             r_norm = self._nrm(self.w)
             if r_norm <= atol or iters >= maxiter:
                 break
-            H = np.zeros((R + 1, R))
             e = np.zeros(R + 1)
             ops.axpby(1.0 / r_norm, self.w, 0.0, self.Vg[0])
             e[0] = r_norm
+            Hd.zero_()
             for j in range(R):
                 z = self.z
                 z.copy_(self.Vg[j])
-                c = self._dot(uu, z)
-                ops.axpby(-c, uu, 1.0, z)
+                self._project(uu, z)
                 self.op.apply(z, self.Az)
                 ops.axpby(-sigma, z, 1.0, self.Az)
                 # the reference subtracts (uu . z) uu here, which is 0 after the projection
-                ops.multidot(self.Vg, j + 1, self.Az, self.h, self.ws)
-                ops.gemv_update(self.Vg, j + 1, self.h, self.Az, 1.0, nrm2_out=self.s1, ws=self.ws)
-                H[:j + 1, j] = self.h[:j + 1].cpu().numpy()
-                H[j + 1, j] = float(self.s1.item())
+                ops.multidot(self.Vg, j + 1, self.Az, Hd[j, :j + 1], self.ws)
+                ops.gemv_update(self.Vg, j + 1, Hd[j, :j + 1], self.Az, 1.0, nrm2_out=Hd[j, j + 1:j + 2], ws=self.ws)
                 if j + 1 < R:
-                    ops.axpby(1.0 / H[j + 1, j] if H[j + 1, j] != 0 else 0.0, self.Az, 0.0, self.Vg[j + 1])
-            y = np.linalg.lstsq(H, e, rcond=None)[0]
-            ops.gemv_update(self.Vg, R, torch.as_tensor(y, device=x.device), x, -1.0)   # x += Vg^T y
+                    ops.scale_div(self.Az, Hd[j, j + 1:j + 2], self.Vg[j + 1])
+            H = Hd.cpu().numpy().T                       # Hd holds the columns of H as rows
+            # a zero subdiagonal entry (breakdown: the Krylov space is invariant) ends the
+            # usable part of the cycle; columns after it were built from a division by zero
+            m = R
+            for j in range(R - 1):
+                if not H[j + 1, j] != 0.0:
+                    m = j + 1
+                    break
+            y = np.linalg.lstsq(H[:m + 1, :m], e[:m + 1], rcond=None)[0]
+            ops.gemv_update(self.Vg, m, torch.as_tensor(y, device=x.device), x, -1.0)   # x += Vg^T y
             iters += R
         return mx
 
