@@ -521,6 +521,8 @@ def kernel_key(kind, A):
     cfg = best_config(A, kind)
     if kind == "mf64":
         c, f = cfg % 1000, cfg // 1000
+        if f & 4 and c > 100:
+            return "k_matfree_clp<double,%s>" % ("true" if f & 2 else "false")
         if f & 4:
             return "k_matfree_cl<double,%d,%s>" % (c, "true" if f & 2 else "false")
         return "k_matfree_epi<double,%d,%d,%s,%s>" % (c // 10, c % 10, "unsigned short" if f & 1 else "int",

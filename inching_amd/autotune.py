@@ -18,7 +18,7 @@ from . import ops
 
 BSR_LANES = (4, 8, 16, 32, 64)
 MATFREE_CONFIGS = (41, 42, 81, 82, 161, 162, 321, 322)
-CLUSTER_CONFIGS = (1, 2, 4)   # list entries per lane iteration
+CLUSTER_CONFIGS = (1, 2, 4, 104, 108, 116)   # list entries per lane iteration; 100 + clusters per wave (persistent)
 
 _cache = {}
 _loaded = False

@@ -159,6 +159,24 @@ __device__ __forceinline__ double recip_r2(double r2) {
   return __builtin_fma(inv, e, inv);
 }
 
+// One neighbour term of a cluster atom: a += d (d . (vi - xj)) / r2 with d = pj - pi, for a
+// set list bit and r2 > 0 (coincident atoms are masked as in the reference). Every rounding
+// is fixed (explicit FMAs), so all forms of the cluster product add bitwise-equal terms.
+// The conditional becomes an exec-masked branch around the reciprocal and the dot, which a
+// wave skips when none of its lanes has the bit (about 6% of the atom blocks on a globule);
+// a straight-line form with selects measured 4% slower.
+__device__ __forceinline__ void cl_term(bool bit, const double (&pi)[3], const double (&vi)[3], double pj0,
+                                        double pj1, double pj2, double xj0, double xj1, double xj2, double *a) {
+  const double d0 = pj0 - pi[0], d1 = pj1 - pi[1], d2 = pj2 - pi[2];
+  const double r2 = __builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0));
+  const double w0 = vi[0] - xj0, w1 = vi[1] - xj1, w2 = vi[2] - xj2;
+  const bool on = bit && r2 > 0.0;
+  const double s = on ? __builtin_fma(d2, w2, __builtin_fma(d1, w1, d0 * w0)) * recip_r2(r2) : 0.0;
+  a[0] = __builtin_fma(d0, s, a[0]);
+  a[1] = __builtin_fma(d1, s, a[1]);
+  a[2] = __builtin_fma(d2, s, a[2]);
+}
+
 // y_i = plus_i x_i + sum_{j != i, r2 > 0} d (d . (x_i - x_j)) / r2,  d = p_j - p_i
 // (the ANM product recomputed from float64 coordinates: H_ij = -d d^T / r2 and
 // H_ii = -sum_j H_ij). Several right-hand sides, row-major [3n, nrhs]: blockIdx.y picks
@@ -309,6 +327,30 @@ __device__ __forceinline__ double swap_sum(double a, double b) {
   }
 }
 
+// Sum over the wave's 64 lanes of a cluster's 12 per-lane partial sums, in a fixed order:
+// two halving exchanges (lanes l and l ^ 32, then l and l ^ 16) leave 3 values per lane
+// (value m + 3 (l / 16) in lane l, summed over l % 16 + 16 k), and 12 lanes add one value
+// each over a 16-lane row through LDS. Lane m < 12 returns value m (atom m / 3, component
+// m % 3); red: 3 x 64 doubles of LDS.
+__device__ __forceinline__ double cl_reduce(const double (&v)[12], int lane, double (*red)[64]) {
+  double u[6], r[3];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) u[m] = swap_sum<32>(v[m], v[m + 6]);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) r[m] = swap_sum<16>(u[m], u[m + 3]);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) red[m][lane] = r[m];
+  __syncthreads();
+  double sum = 0.0;
+  if (lane < 12) {
+    const int row = lane / 3, m = lane % 3;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) sum += red[m][16 * row + q];
+  }
+  __syncthreads();
+  return sum;
+}
+
 // Raw buffer loads with 32-bit byte offsets from a descriptor built per array (word 3 of
 // the gfx9-family descriptor is 0x00020000): no 64-bit address arithmetic per gather, and
 // a load at or past num_records bytes returns 0 (hardware range check).
@@ -344,8 +386,8 @@ k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32
   if (c >= ncl) return;
   const int lane = threadIdx.x;
   double pi[kClAtoms][3], vi[kClAtoms][3];
-  __shared__ double red[3 * kClAtoms][65];  // cross-lane reduction (6.2 KB per wave)
-  double v[3 * kClAtoms];                   // 4 atoms x 3 partial sums
+  __shared__ double red[3][64];  // cross-lane reduction
+  double v[3 * kClAtoms];        // 4 atoms x 3 partial sums
 #pragma unroll
   for (int t = 0; t < kClAtoms; ++t) {
     const int32_t at = cluster_atoms[kClAtoms * c + t];
@@ -400,35 +442,12 @@ k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32
       const double pj0 = pj[q][0], pj1 = pj[q][1], pj2 = pj[q][2];
       const double xj0 = xj[q][0], xj1 = xj[q][1], xj2 = xj[q][2];
 #pragma unroll
-      for (int t = 0; t < kClAtoms; ++t) {
-        const double d0 = pj0 - pi[t][0], d1 = pj1 - pi[t][1], d2 = pj2 - pi[t][2];
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        const double w0 = vi[t][0] - xj0, w1 = vi[t][1] - xj1, w2 = vi[t][2] - xj2;
-        // r2 == 0 (coincident atoms) is masked as in the reference
-        const bool on = ((w[q] >> t) & 1u) && r2 > 0.0;
-        const double s = on ? (d0 * w0 + d1 * w1 + d2 * w2) * recip_r2(r2) : 0.0;
-        v[3 * t] += d0 * s;
-        v[3 * t + 1] += d1 * s;
-        v[3 * t + 2] += d2 * s;
-      }
+      for (int t = 0; t < kClAtoms; ++t)
+        cl_term((w[q] >> t) & 1u, pi[t], vi[t], pj0, pj1, pj2, xj0, xj1, xj2, v + 3 * t);
     }
   }
-  // cross-lane sums through LDS in a fixed order: lanes 0..47 add 16 lanes' copies of value
-  // lane % 12 (quarter lane / 12), lanes 0..11 add the four quarters
-#pragma unroll
-  for (int m = 0; m < 3 * kClAtoms; ++m) red[m][lane] = v[m];
-  __syncthreads();
-  double sum = 0.0;
-  if (lane < 48) {
-    const int m = lane % 12, h = lane / 12;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) sum += red[m][16 * h + q];
-  }
-  __syncthreads();
-  red[0][lane] = sum;
-  __syncthreads();
+  const double sum = cl_reduce(v, lane, red);
   if (at_w < 0) return;  // lanes >= 12 and padding slots
-  sum = red[0][lane] + red[0][lane + 12] + red[0][lane + 24] + red[0][lane + 36];
   double out = sum + plus_i * xv;
   if (c0 != 0.0) out -= c0 * xv;
   out *= s1;
@@ -441,6 +460,119 @@ k_matfree_cl(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32
   }
   y[dof] = (VT)out;
   if (acc) acc[dof] = (VT)(av + mu * out);
+}
+
+// Persistent form of the cluster product: one wave takes cpw (<= 16) consecutive clusters.
+// Their list pointers, bases and atom slots are loaded once (lane l holds the pointer and
+// base of the wave's cluster l and its atom slot l), and the first list entries of the next
+// cluster are loaded while the last list chunk of the current one is computed. A cluster's
+// coordinates and vector rows (scalar loads) and its first neighbour gathers are then in
+// flight together, so a wave waits on memory once per list chunk; the per-cluster form
+// waits three times before its first chunk (list pointer, list entries, gathers). The
+// clusters' sums wait in LDS and the epilogue runs once for all of them after the loop,
+// 64 values per pass. Per lane the terms are added in the order of k_matfree_cl and the
+// reduction is the same, so the two forms give bitwise-equal results.
+constexpr int kClpMax = 16;  // clusters per wave
+
+template <typename VT, bool XCD>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(VT) == 8 ? 6 : 1)))
+k_matfree_clp(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int32_t *__restrict__ cluster_atoms,
+              const int64_t *__restrict__ cl_ptr, const int32_t *__restrict__ base,
+              const uint32_t *__restrict__ entries, double plus_i, const VT *__restrict__ x, VT *__restrict__ y,
+              double s1, double c0, const VT *__restrict__ z, double c2, VT *__restrict__ acc, double mu,
+              const double *__restrict__ Q, const double *__restrict__ qcoef, int mq, int cpw) {
+  const int64_t cw = xcd_block(XCD) * (int64_t)cpw;
+  if (cw >= ncl) return;
+  const int nc = (int)(ncl - cw < cpw ? ncl - cw : cpw);
+  const int lane = threadIdx.x;
+  __shared__ double red[3][64];
+  __shared__ double sums[kClpMax * 3 * kClAtoms];  // value m of the wave's cluster ci at 12 ci + m
+  // the wave's header: lane l <= nc holds cl_ptr[cw + l], lane l < nc base[cw + l],
+  // lane l < 4 nc the atom of slot l (clusters cw .. cw + nc - 1)
+  const int64_t hp = lane <= nc ? cl_ptr[cw + lane] : 0;
+  const int32_t hb = lane < nc ? base[cw + lane] : 0;
+  const int32_t ha = lane < kClAtoms * nc ? cluster_atoms[kClAtoms * cw + lane] : -1;
+  const int hp_lo = (int)(uint32_t)hp, hp_hi = (int)(uint32_t)((uint64_t)hp >> 32);
+  auto ptr_at = [&](int l) -> int64_t {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readlane(hp_hi, l) << 32) |
+                     (uint64_t)(uint32_t)__builtin_amdgcn_readlane(hp_lo, l));
+  };
+  const __amdgpu_buffer_rsrc_t rp = buf_rsrc(xyz, (uint32_t)(24 * n));
+  const __amdgpu_buffer_rsrc_t rx = buf_rsrc(x, (uint32_t)(3 * sizeof(VT) * n));
+  // the wave's lists are one range of at most 16 x 4096 entries: 32-bit offsets from its
+  // start; an offset past the range (kOut) reads 0, an entry without a term
+  const int64_t wstart = ptr_at(0);
+  const __amdgpu_buffer_rsrc_t rw = buf_rsrc(entries + wstart, (uint32_t)(4 * (ptr_at(nc) - wstart)));
+  constexpr uint32_t kOut = 0xfffffff0u;
+  uint32_t off = 0, len = (uint32_t)(ptr_at(1) - wstart);  // len < 4096 (inch_anm_clusters)
+  uint32_t wn = __builtin_amdgcn_raw_buffer_load_b32(rw, (int)((uint32_t)lane < len ? 4u * lane : kOut), 0, 0);
+  for (int ci = 0; ci < nc; ++ci) {
+    const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane(hb, ci);
+    // the next cluster's list (empty after the wave's last cluster)
+    const uint32_t off_n = off + len;
+    const uint32_t len_n = ci + 1 < nc ? (uint32_t)(ptr_at(ci + 2) - wstart) - off_n : 0u;
+    double pi[kClAtoms][3], vi[kClAtoms][3];
+#pragma unroll
+    for (int t = 0; t < kClAtoms; ++t) {
+      const int32_t at = __builtin_amdgcn_readlane(ha, kClAtoms * ci + t);
+      const int64_t i = at >= 0 ? at : 0;  // a padding slot has no list bits
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        pi[t][k] = xyz[3 * i + k];
+        vi[t][k] = (double)x[3 * i + k];
+      }
+    }
+    double v[3 * kClAtoms];
+#pragma unroll
+    for (int q = 0; q < 3 * kClAtoms; ++q) v[q] = 0.0;
+    for (uint32_t e0 = 0; e0 < len; e0 += 64) {
+      const uint32_t w = wn;
+      const uint32_t j = cb + (w >> 8);
+      double pj[3], xj[3];
+      load3(rp, 24u * j, xyz, pj);
+      load3(rx, (uint32_t)(3 * sizeof(VT)) * j, x, xj);
+      if (e0 + 64 < len) {
+        const uint32_t q = e0 + 64 + lane;
+        wn = __builtin_amdgcn_raw_buffer_load_b32(rw, (int)(q < len ? 4u * (off + q) : kOut), 0, 0);
+      } else {
+        wn = __builtin_amdgcn_raw_buffer_load_b32(rw, (int)((uint32_t)lane < len_n ? 4u * (off_n + lane) : kOut),
+                                                  0, 0);
+      }
+      // the gathers and the next entries are in flight before any use
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < kClAtoms; ++t)
+        cl_term((w >> t) & 1u, pi[t], vi[t], pj[0], pj[1], pj[2], xj[0], xj[1], xj[2], v + 3 * t);
+    }
+    if (len == 0)
+      wn = __builtin_amdgcn_raw_buffer_load_b32(rw, (int)((uint32_t)lane < len_n ? 4u * (off_n + lane) : kOut), 0, 0);
+    const double sum = cl_reduce(v, lane, red);
+    if (lane < 3 * kClAtoms) sums[3 * kClAtoms * ci + lane] = sum;
+    off = off_n;
+    len = len_n;
+  }
+  __syncthreads();
+  // epilogue: value q = 12 ci + m of the wave is atom slot 4 ci + m / 3, component m % 3
+  const int64_t nd = 3 * n;
+  for (int q0 = 0; q0 < 3 * kClAtoms * nc; q0 += 64) {
+    const int q = q0 + lane;
+    const int ci = q / (3 * kClAtoms), m = q % (3 * kClAtoms);
+    const int32_t at = __builtin_amdgcn_ds_bpermute(4 * (kClAtoms * ci + m / 3), ha);
+    if (q >= 3 * kClAtoms * nc || at < 0) continue;
+    const int64_t dof = 3 * (int64_t)at + m % 3;
+    const double xv = (double)x[dof];
+    double out = sums[q] + plus_i * xv;
+    if (c0 != 0.0) out -= c0 * xv;
+    out *= s1;
+    if (z) out -= c2 * (double)z[dof];
+    if (qcoef) {
+      double qs = 0.0;
+      for (int mm = 0; mm < mq; ++mm) qs += qcoef[mm] * Q[mm * nd + dof];
+      out += qs;
+    }
+    y[dof] = (VT)out;
+    if (acc) acc[dof] = (VT)((double)acc[dof] + mu * out);
+  }
 }
 
 // Several right-hand sides on the cluster layout: Y = (H + plus_i I) X, X and Y row-major
@@ -1027,6 +1159,16 @@ void launch_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const int32_t 
                                                          mu, Q, qcoef, m_q);
 }
 
+template <typename VT, bool XCD>
+void launch_matfree_clp(int cpw, int64_t n, const double *xyz, int64_t ncl, const int32_t *catoms,
+                        const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double plus_i,
+                        const void *x, void *y, double s1, double c0, const void *z, double c2, void *acc, double mu,
+                        const double *Q, const double *qcoef, int m_q, hipStream_t st) {
+  k_matfree_clp<VT, XCD><<<(unsigned)((ncl + cpw - 1) / cpw), 64, 0, st>>>(
+      n, xyz, ncl, catoms, cl_ptr, base, entries, plus_i, (const VT *)x, (VT *)y, s1, c0, (const VT *)z, c2, (VT *)acc,
+      mu, Q, qcoef, m_q, cpw);
+}
+
 template <typename VT>
 bool dispatch_matfree_cl(int code, int64_t n, const double *xyz, int64_t ncl, const int32_t *catoms,
                          const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double plus_i,
@@ -1046,9 +1188,21 @@ bool dispatch_matfree_cl(int code, int64_t n, const double *xyz, int64_t ncl, co
     INCH_CL(2)
     INCH_CL(4)
     default:
-      return false;
+      break;
   }
 #undef INCH_CL
+  // 100 + cpw: the persistent form, cpw clusters per wave
+  const int shape = code % 2000, cpw = shape - 100;
+  if (code < 4000 && (cpw == 4 || cpw == 8 || cpw == 16)) {
+    if (code >= 2000)
+      launch_matfree_clp<VT, true>(cpw, n, xyz, ncl, catoms, cl_ptr, base, entries, plus_i, x, y, s1, c0, z, c2, acc,
+                                   mu, Q, qcoef, m_q, st);
+    else
+      launch_matfree_clp<VT, false>(cpw, n, xyz, ncl, catoms, cl_ptr, base, entries, plus_i, x, y, s1, c0, z, c2, acc,
+                                    mu, Q, qcoef, m_q, st);
+    return true;
+  }
+  return false;
 }
 
 }  // namespace
@@ -1241,8 +1395,8 @@ inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const
                                     z, c2, acc, mu, Q, qcoef, m_q, st);
   else
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: bad dtype");
-  if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: config %d is not an unroll in {1,2,4}, plus 2000 for "
-                                       "XCD-contiguous clusters", code);
+  if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: config %d is not an unroll in {1,2,4} or 100 + clusters "
+                                       "per wave in {4,8,16}, plus 2000 for XCD-contiguous clusters", code);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

@@ -225,8 +225,10 @@ inch_status inch_anm_clusters(int64_t n, const double *xyz, const int64_t *rowpt
                               int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out, void *stream);
 /* inch_anm_matfree_spmv on the cluster layout: the same product and fused epilogue, the
  * neighbour's coordinates and vector entries loaded once per cluster (one wave per cluster,
- * lanes over its list). config = list entries per lane iteration (1, 2 or 4; 0 = 2002),
- * + 2000 maps each XCD's workgroups to one contiguous range of clusters. */
+ * lanes over its list). config = list entries per lane iteration (1, 2 or 4; 0 = 2002), or
+ * 100 + clusters per wave (4, 8 or 16) for the persistent form, which prefetches the next
+ * cluster's list while computing the current one (bitwise-equal results); + 2000 maps each
+ * XCD's workgroups to one contiguous range of clusters. */
 inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
                                 const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double plus_i,
                                 int32_t vdt, const void *x, void *y, double s1, double c0, const void *z, double c2,

@@ -522,6 +522,46 @@ def test_cluster_product_matches_oracle(mat, glob2k):
     assert np.linalg.norm(y32.cpu().numpy() - ref32) <= 1e-6 * np.linalg.norm(ref32)
 
 
+@pytest.mark.parametrize("case", ["globule2k", "padding_isolated"])
+def test_persistent_cluster_product_bitwise(mat, case):
+    """The persistent cluster kernel (4, 8 or 16 clusters per wave, plain and XCD-contiguous)
+    against the per-cluster kernel: bitwise-equal outputs and accumulators with every
+    epilogue term, float64 and float32 vectors; a globule of 2000 atoms, and 1003 atoms plus
+    8 isolated ones (a padding slot, clusters with empty lists, a partial last wave)."""
+    from inching_amd.hessian import build_hessian, rigid_body_basis
+    if case == "globule2k":
+        A, _ = mat
+    else:
+        X = small_globule(1003, seed=23)
+        far = np.stack([np.full(8, 500.0), np.full(8, 500.0), 60.0 * np.arange(8)], axis=1)
+        A = build_hessian(np.concatenate([X, far]), 8.0, plus_i=1.0, reorder=True, precision=())
+        assert A.clusters() is not None
+        ptr = A.clusters()[1].cpu().numpy()
+        assert (np.diff(ptr) == 0).any()          # some clusters hold only isolated atoms
+        assert (A.clusters()[0] == -1).any()      # and one a padding slot (1011 atoms)
+    nd = A.shape[0]
+    _print("case: persistent cluster product, %s, n=%d" % (case, A.n_atoms))
+    Q = rigid_body_basis(A.xyz)
+    rng = np.random.default_rng(77)
+    x, z, acc0 = (rng.standard_normal(nd) for _ in range(3))
+    qc = rng.standard_normal(Q.shape[0])
+    for dt in (torch.float64, torch.float32):
+        epis = ({}, dict(s1=0.3, c0=2.0, z=_dev(z, dt), c2=1.0, mu=0.5),
+                dict(s1=0.3, c0=2.0, z=_dev(z, dt), c2=1.0, mu=0.5, Q=Q, qcoef=_dev(qc)))
+        for k, epi in enumerate(epis):
+            with_acc = "mu" in epi
+            ref = torch.empty(nd, dtype=dt, device="cuda")
+            ref_acc = _dev(acc0, dt) if with_acc else None
+            A.matfree_spmv(_dev(x, dt), ref, config=4001, **(dict(epi, acc=ref_acc) if with_acc else epi))
+            for cfg in (4104, 4108, 4116, 6104, 6108, 6116):
+                y = torch.full((nd,), float("nan"), dtype=dt, device="cuda")
+                acc = _dev(acc0, dt) if with_acc else None
+                A.matfree_spmv(_dev(x, dt), y, config=cfg, **(dict(epi, acc=acc) if with_acc else epi))
+                assert torch.equal(y, ref), (cfg, dt, k)
+                if with_acc:
+                    assert torch.equal(acc, ref_acc), (cfg, dt, k)
+
+
 def test_cluster_capacity_falls_back():
     """A cluster whose rows hold more than 4096 entries makes the layout unavailable (a host
     error, no kernel fault); the operator then keeps the per-row kernels."""
