@@ -172,8 +172,10 @@ __global__ void k_neigh(int64_t n, const double *__restrict__ xyz, Grid g, doubl
                         const int32_t *__restrict__ start, const int32_t *__restrict__ atoms,
                         const int64_t *__restrict__ indptr, int64_t *__restrict__ counts,
                         int32_t *__restrict__ indices) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  // threads take the atoms in cell order, so that the lanes of a wave scan the same cells
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = atoms[t];
   const double xi = xyz[3 * i], yi = xyz[3 * i + 1], zi = xyz[3 * i + 2];
   int64_t c0 = cell_coord(xi, g.lo[0], g.inv, g.dim[0]);
   int64_t c1 = cell_coord(yi, g.lo[1], g.inv, g.dim[1]);
@@ -262,8 +264,9 @@ __global__ void k_neigh_pbc(int64_t n, const double *__restrict__ xyz, PbcGrid g
                             const int32_t *__restrict__ start, const int32_t *__restrict__ atoms,
                             const int64_t *__restrict__ indptr, int64_t *__restrict__ counts,
                             int32_t *__restrict__ indices) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;   // atoms in cell order
+  if (t >= n) return;
+  const int64_t i = atoms[t];
   const int64_t c0 = pcell(xyz[3 * i], g.lo[0], g.box[0], g.cell[0], g.dim[0]);
   const int64_t c1 = pcell(xyz[3 * i + 1], g.lo[1], g.box[1], g.cell[1], g.dim[1]);
   const int64_t c2 = zcell(xyz[3 * i + 2], g);

@@ -11,7 +11,7 @@
 //   count   : cnt[p]   = number of nodes claimed by the node at position p
 //   scan    : off      = exclusive prefix sum of cnt (single workgroup)
 //   scatter : children written into their parent's slot range        (atomic slot)
-//   sort    : each group sorted by (degree, -index) (one thread per parent)
+//   sort    : each group sorted by (degree, -index) (one wave per parent, rank by key)
 //   place   : pos[node] = its position
 // The level size comes back to the host after the scan; a new component starts from the
 // first unvisited node in the caller's seed order.
@@ -134,25 +134,52 @@ __global__ void k_rcm_scatter(int32_t s, int32_t e, int32_t *__restrict__ ord, c
   }
 }
 
-// Children of one parent: ascending degree, then descending index (one thread per parent).
-__global__ void k_rcm_sort_groups(int32_t L, int32_t e, int32_t *__restrict__ ord, const int64_t *__restrict__ indptr,
-                                  const int32_t *__restrict__ off) {
-  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+// Children of one parent: ascending degree, then descending index. One wave per parent:
+// each child's key (degree << 32 | ~index) goes to LDS and its rank among the group's keys
+// (all distinct) is its slot, which is the order a stable insertion sort by the same rule
+// gives. Groups above kSortCap children (not produced by the neighbour lists, whose rows
+// hold at most 2048 entries) are sorted by insertion in lane 0.
+constexpr int kSortCap = 2048;
+
+__global__ void __launch_bounds__(64) k_rcm_sort_groups(int32_t L, int32_t e, int32_t *__restrict__ ord,
+                                                        const int64_t *__restrict__ indptr,
+                                                        const int32_t *__restrict__ off) {
+  __shared__ uint64_t key[kSortCap];
+  const int32_t q = blockIdx.x;
+  const int lane = threadIdx.x;
   if (q >= L) return;
   int32_t *g = ord + e + off[q];
   const int32_t m = off[q + 1] - off[q];
-  for (int32_t a = 1; a < m; ++a) {
-    const int32_t v = g[a];
-    const int64_t dv = indptr[v + 1] - indptr[v];
-    int32_t b = a;
-    while (b > 0) {
-      const int32_t u = g[b - 1];
-      const int64_t du = indptr[u + 1] - indptr[u];
-      if (du < dv || (du == dv && u > v)) break;
-      g[b] = u;
-      --b;
+  if (m <= 1) return;
+  if (m > kSortCap) {
+    if (lane == 0) {
+      for (int32_t a = 1; a < m; ++a) {
+        const int32_t v = g[a];
+        const int64_t dv = indptr[v + 1] - indptr[v];
+        int32_t b = a;
+        while (b > 0) {
+          const int32_t u = g[b - 1];
+          const int64_t du = indptr[u + 1] - indptr[u];
+          if (du < dv || (du == dv && u > v)) break;
+          g[b] = u;
+          --b;
+        }
+        g[b] = v;
+      }
     }
-    g[b] = v;
+    return;
+  }
+  for (int32_t a = lane; a < m; a += 64) {
+    const int32_t v = g[a];
+    key[a] = ((uint64_t)(indptr[v + 1] - indptr[v]) << 32) | (uint64_t)(0xffffffffu - (uint32_t)v);
+  }
+  __syncthreads();
+  for (int32_t a0 = 0; a0 < m; a0 += 64) {
+    const int32_t a = a0 + lane;
+    const uint64_t ka = a < m ? key[a] : 0;
+    int32_t r = 0;
+    for (int32_t b = 0; b < m; ++b) r += key[b] < ka;
+    if (a < m) g[r] = (int32_t)(0xffffffffu - (uint32_t)ka);
   }
 }
 
@@ -308,7 +335,7 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
       if (L1 == 0) break;
       k_rcm_scatter<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim, off,
                                                                     fill);
-      k_rcm_sort_groups<<<blocks_for(L, bs), bs, 0, st>>>(L, e, ord, indptr, off);
+      k_rcm_sort_groups<<<(unsigned)L, 64, 0, st>>>(L, e, ord, indptr, off);
       k_rcm_place<<<blocks_for(L1, bs), bs, 0, st>>>(e, L1, ord, pos);
       HIP_TRY(hipGetLastError());
       s = e;
