@@ -12,6 +12,9 @@
 #include <vector>
 
 #include "common.h"
+#include "scan.h"
+
+#include <hipcub/hipcub.hpp>
 
 namespace {
 
@@ -79,84 +82,6 @@ __global__ void k_cell_count(int64_t n, const double *__restrict__ xyz, Grid g, 
   atomicAdd(&count[id], 1);
 }
 
-// Exclusive scans (out[i] = sum in[0..i-1], out[m] = total). One block of 1024 threads
-// walks short arrays in chunks; longer ones take three passes: per-tile sums (kScanTile
-// elements per 256-thread block), the single-block scan of the tile sums, and a rescan of
-// every tile from its offset.
-template <typename T>
-__global__ void k_scan_one(int64_t m, const T *__restrict__ in, T *__restrict__ out) {
-  __shared__ T buf[1024];
-  __shared__ T carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < m; base += 1024) {
-    const int64_t i = base + threadIdx.x;
-    const T v = i < m ? in[i] : 0;
-    buf[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      const T t = (int)threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
-      __syncthreads();
-      buf[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < m) out[i] = carry + buf[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += buf[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[m] = carry;
-}
-
-constexpr int kScanPer = 16;                 // elements per thread of a tile
-constexpr int kScanTile = 256 * kScanPer;    // elements per tile (one 256-thread block)
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_scan_tile_sums(int64_t m, const T *__restrict__ in, T *__restrict__ sums) {
-  __shared__ T red[256];
-  const int64_t b0 = blockIdx.x * (int64_t)kScanTile;
-  T s = 0;
-  for (int q = threadIdx.x; q < kScanTile; q += 256) {
-    const int64_t i = b0 + q;
-    if (i < m) s += in[i];
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 128; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_scan_tiles(int64_t m, const T *__restrict__ in, const T *__restrict__ offs,
-                                                    T *__restrict__ out) {
-  __shared__ T ts[256];
-  const int64_t i0 = blockIdx.x * (int64_t)kScanTile + threadIdx.x * (int64_t)kScanPer;
-  T v[kScanPer];
-  T tot = 0;
-#pragma unroll
-  for (int q = 0; q < kScanPer; ++q) {
-    v[q] = i0 + q < m ? in[i0 + q] : 0;
-    tot += v[q];
-  }
-  ts[threadIdx.x] = tot;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const T t = (int)threadIdx.x >= off ? ts[threadIdx.x - off] : 0;
-    __syncthreads();
-    ts[threadIdx.x] += t;
-    __syncthreads();
-  }
-  T run = offs[blockIdx.x] + ts[threadIdx.x] - tot;
-#pragma unroll
-  for (int q = 0; q < kScanPer; ++q) {
-    if (i0 + q < m) out[i0 + q] = run;
-    run += v[q];
-  }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[m] = offs[gridDim.x];
-}
 
 __global__ void k_cell_fill(int64_t n, const int32_t *__restrict__ cell_of, const int32_t *__restrict__ start,
                             int32_t *__restrict__ cursor, int32_t *__restrict__ atoms) {
@@ -642,14 +567,21 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const int64_t *__
     if (val[e]) entries[out + r++] = (((uint32_t)(key[e] >> 3) - b0) << 8) | (uint32_t)val[e];
 }
 
-// Clusters renumbered: slot c' takes cluster perm[c'] (atoms, list length, base).
-__global__ void k_cl_permute(int64_t ncl, const int32_t *__restrict__ perm, const int32_t *__restrict__ atoms_in,
+// Sort keys of the clusters: smallest neighbour in the high word, cluster index in the low.
+__global__ void k_cl_keys(int64_t ncl, const int32_t *__restrict__ base, uint64_t *__restrict__ key) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c < ncl) key[c] = ((uint64_t)(uint32_t)base[c] << 32) | (uint64_t)c;
+}
+
+// Clusters renumbered: slot c' takes cluster perm[c'] = low word of the c'-th sorted key
+// (atoms, list length, base).
+__global__ void k_cl_permute(int64_t ncl, const uint64_t *__restrict__ perm, const int32_t *__restrict__ atoms_in,
                              const int64_t *__restrict__ counts_in, const int32_t *__restrict__ base_in,
                              int32_t *__restrict__ atoms_out, int64_t *__restrict__ counts_out,
                              int32_t *__restrict__ base_out) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= ncl) return;
-  const int64_t src = perm[c];
+  const int64_t src = (int64_t)(perm[c] & 0xffffffffu);
   for (int t = 0; t < kClusterAtoms; ++t) atoms_out[kClusterAtoms * c + t] = atoms_in[kClusterAtoms * src + t];
   counts_out[c] = counts_in[src];
   base_out[c] = base_in[src];
@@ -672,26 +604,18 @@ struct DevBuf {
 
 namespace {
 
-// out[0..m] = exclusive scan of in[0..m-1] (out[m] = total), on stream st.
+// out[0..m] = exclusive scan of in[0..m-1] (out[m] = total), on stream st; the tile
+// temporaries are allocated here and freed after the scan has finished.
 template <typename T>
 inch_status scan_exclusive(int64_t m, const T *in, T *out, hipStream_t st) {
-  const int64_t nt = (m + kScanTile - 1) / kScanTile;
-  if (nt <= 4) {
-    k_scan_one<T><<<1, 1024, 0, st>>>(m, in, out);
-    HIP_TRY(hipGetLastError());
-    return INCH_OK;
-  }
+  const int64_t nt = scan_tiles(m);
   DevBuf sums, offs;
-  HIP_TRY(hipMalloc(&sums.p, nt * sizeof(T)));
-  HIP_TRY(hipMalloc(&offs.p, (nt + 1) * sizeof(T)));
-  k_scan_tile_sums<T><<<(unsigned)nt, 256, 0, st>>>(m, in, (T *)sums.p);
-  HIP_TRY(hipGetLastError());
-  k_scan_one<T><<<1, 1024, 0, st>>>(nt, (const T *)sums.p, (T *)offs.p);
-  HIP_TRY(hipGetLastError());
-  k_scan_tiles<T><<<(unsigned)nt, 256, 0, st>>>(m, in, (const T *)offs.p, out);
-  HIP_TRY(hipGetLastError());
-  // the temporaries are freed when this returns: wait for the kernels that read them
-  HIP_TRY(hipStreamSynchronize(st));
+  if (nt > 4) {
+    HIP_TRY(hipMalloc(&sums.p, (nt + 1) * sizeof(T)));
+    HIP_TRY(hipMalloc(&offs.p, (nt + 1) * sizeof(T)));
+  }
+  HIP_TRY(scan_exclusive_into(m, in, out, (T *)sums.p, (T *)offs.p, st));
+  if (nt > 4) HIP_TRY(hipStreamSynchronize(st));
   return INCH_OK;
 }
 
@@ -1017,7 +941,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   HIP_TRY(hipMalloc(&atoms0.p, slots * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&counts0.p, ncl * sizeof(int64_t)));
   HIP_TRY(hipMalloc(&base0.p, ncl * sizeof(int32_t)));
-  HIP_TRY(hipMalloc(&perm.p, ncl * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&perm.p, ncl * sizeof(uint64_t)));
   HIP_TRY(hipMemcpyAsync(atoms0.p, cluster_atoms, slots * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
   k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
                                                    (int64_t *)counts0.p, nullptr, (int32_t *)ovf.p);
@@ -1028,16 +952,18 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   }
   // clusters in the order of their smallest neighbour (ties by spatial order): consecutive
   // clusters, and so each XCD's contiguous share of them, gather from one band of the
-  // vectors, which are in the matrix's (RCM) order
-  std::vector<int32_t> hb(ncl), order(ncl);
-  HIP_TRY(hipMemcpyAsync(hb.data(), base0.p, ncl * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  std::vector<uint64_t> key(ncl);
-  for (int64_t c = 0; c < ncl; ++c) key[c] = ((uint64_t)(uint32_t)hb[c] << 32) | (uint64_t)c;
-  std::sort(key.begin(), key.end());
-  for (int64_t c = 0; c < ncl; ++c) order[c] = (int32_t)(key[c] & 0xffffffffu);
-  HIP_TRY(hipMemcpyAsync(perm.p, order.data(), ncl * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  k_cl_permute<<<(unsigned)((ncl + bs - 1) / bs), bs, 0, st>>>(ncl, (int32_t *)perm.p, (int32_t *)atoms0.p,
+  // vectors, which are in the matrix's (RCM) order. Radix sort of the keys on the device.
+  DevBuf keys, tmp;
+  HIP_TRY(hipMalloc(&keys.p, ncl * sizeof(uint64_t)));
+  k_cl_keys<<<(unsigned)((ncl + bs - 1) / bs), bs, 0, st>>>(ncl, (int32_t *)base0.p, (uint64_t *)keys.p);
+  HIP_TRY(hipGetLastError());
+  size_t tmp_bytes = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, (const uint64_t *)keys.p, (uint64_t *)perm.p,
+                                            (int)ncl, 0, 64, st));
+  HIP_TRY(hipMalloc(&tmp.p, tmp_bytes));
+  HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp.p, tmp_bytes, (const uint64_t *)keys.p, (uint64_t *)perm.p, (int)ncl,
+                                            0, 64, st));
+  k_cl_permute<<<(unsigned)((ncl + bs - 1) / bs), bs, 0, st>>>(ncl, (const uint64_t *)perm.p, (int32_t *)atoms0.p,
                                                               (int64_t *)counts0.p, (int32_t *)base0.p, cluster_atoms,
                                                               (int64_t *)counts.p, base);
   HIP_TRY(hipGetLastError());

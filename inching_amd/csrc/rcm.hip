@@ -3,24 +3,32 @@
 //
 // The sequential algorithm appends, for each dequeued node in queue order, its unvisited
 // neighbours (scanned in descending index order) and sorts those new children stably by
-// degree. Queue order is level order, so level L+1 consists of the unvisited neighbours of
-// level L, each one the child of the first level-L node (lowest queue position) adjacent to
-// it, and inside one parent's group the children are ordered by (degree ascending, index
-// descending). One level is built per round:
+// degree; when the queue runs dry, the first unvisited node in the caller's seed order
+// starts the next connected component. Queue order is level order, so level L+1 consists of
+// the unvisited neighbours of level L, each one the child of the first level-L node (lowest
+// queue position) adjacent to it, and inside one parent's group the children are ordered by
+// (degree ascending, index descending). One level is built per round:
 //   claim   : claim[j] = min position of a level-L neighbour           (atomicMin)
 //   count   : cnt[p]   = number of nodes claimed by the node at position p
-//   scan    : off      = exclusive prefix sum of cnt (single workgroup)
+//   scan    : off      = exclusive prefix sum of cnt
 //   scatter : children written into their parent's slot range        (atomic slot)
 //   sort    : each group sorted by (degree, -index) (one wave per parent, rank by key)
 //   place   : pos[node] = its position
-// The level size comes back to the host after the scan; a new component starts from the
-// first unvisited node in the caller's seed order.
+// Components do not interact, so all of them advance in the same rounds: the components
+// are found first (union-find), each one's seed is its first node in the seed order, and
+// the components, ordered by their seeds' places in the seed order, are the order in which
+// the sequential algorithm visits them. A round's frontier holds the components' levels one
+// after another in that order, so inside each component positions compare as in the
+// sequential run; every node's final position is its component's start plus its rank in
+// the component (levels before its own, then its place in its own level). The number of
+// rounds is the deepest component's level count, not the sum over components.
 #include <hip/hip_runtime.h>
 
 #include <climits>
 #include <vector>
 
 #include "common.h"
+#include "scan.h"
 
 #define HIP_TRY(expr)                                                                      \
   do {                                                                                     \
@@ -76,44 +84,6 @@ __global__ void k_rcm_count(int32_t s, int32_t e, const int32_t *__restrict__ or
     if (pos[j] < 0 && claim[j] == (int32_t)p) ++c;
   }
   if (c) atomicAdd(&cnt[p - s], c);
-}
-
-// off[0..L] = exclusive prefix sum of cnt[0..L); off[L] = total. One workgroup of 1024.
-__global__ void __launch_bounds__(1024) k_rcm_scan(int32_t L, const int32_t *__restrict__ cnt,
-                                                   int32_t *__restrict__ off) {
-  __shared__ int32_t wsum[16];
-  __shared__ int32_t carry_s;
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  if (t == 0) carry_s = 0;
-  __syncthreads();
-  for (int32_t base = 0; base < L; base += 1024) {
-    const int32_t v = (base + t < L) ? cnt[base + t] : 0;
-    int32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int32_t y = __shfl_up(x, d, 64);
-      if (lane >= d) x += y;
-    }
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    if (w == 0) {
-      int32_t ws = lane < 16 ? wsum[lane] : 0;
-#pragma unroll
-      for (int d = 1; d < 16; d <<= 1) {
-        const int32_t y = __shfl_up(ws, d, 64);
-        if (lane >= d) ws += y;
-      }
-      if (lane < 16) wsum[lane] = ws;   // inclusive prefix of wave totals
-    }
-    __syncthreads();
-    const int32_t carry = carry_s;
-    const int32_t incl = x + (w > 0 ? wsum[w - 1] : 0);
-    if (base + t < L) off[base + t] = carry + incl - v;
-    __syncthreads();
-    if (t == 1023) carry_s = carry + incl;
-    __syncthreads();
-  }
-  if (t == 0) off[L] = carry_s;
 }
 
 __global__ void k_rcm_scatter(int32_t s, int32_t e, int32_t *__restrict__ ord, const int64_t *__restrict__ indptr,
@@ -188,47 +158,172 @@ __global__ void k_rcm_place(int32_t e, int32_t L1, const int32_t *__restrict__ o
   if (q < L1) pos[ord[e + q]] = e + q;
 }
 
-// First k >= z with pos[seeds[k]] < 0 (atomicMin into *best, ULLONG_MAX when none). The
-// block's minimum is formed first (wave shuffles, then the block's waves), so a block does
-// at most one atomic and blocks with no unvisited seed none: with many components almost
-// every entry past z is still unvisited, and one atomic per entry serialised on *best.
-__global__ void __launch_bounds__(256) k_rcm_next_seed(int64_t n, int64_t z, const int64_t *__restrict__ seeds,
-                                                       const int32_t *__restrict__ pos,
-                                                       unsigned long long *__restrict__ best) {
-  __shared__ unsigned long long wmin[4];
-  const int64_t k = z + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  unsigned long long cand = (k < n && pos[seeds[k]] < 0) ? (unsigned long long)k : ULLONG_MAX;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(cand, off, 64);
-    cand = o < cand ? o : cand;
+// ---- components: union-find over the edges. A root is only ever hooked (CAS) onto a
+// smaller root, so parent[v] <= v throughout and each component ends with its smallest node
+// as root; finds shorten the paths they walk (the ECL-CC scheme, Jaiganesh and Burtscher,
+// 2018). Loads and stores of parent are device-coherent: other workgroups change it. ----
+__device__ __forceinline__ int32_t cc_load(const int32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void cc_store(int32_t *p, int32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int32_t cc_find(int32_t *parent, int32_t v) {
+  int32_t par = cc_load(parent + v);
+  if (par != v) {
+    int32_t prev = v, next;
+    while (par > (next = cc_load(parent + par))) {
+      cc_store(parent + prev, next);
+      prev = par;
+      par = next;
+    }
   }
-  if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = cand;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long m = wmin[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = wmin[w] < m ? wmin[w] : m;
-    if (m != ULLONG_MAX) atomicMin(best, m);
+  return par;
+}
+
+__global__ void k_cc_init(int64_t n, int32_t *__restrict__ parent) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v < n) parent[v] = (int32_t)v;
+}
+
+__global__ void k_cc_hook(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+                          int32_t *parent) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int32_t vr = cc_find(parent, (int32_t)v);
+  for (int64_t b = indptr[v]; b < indptr[v + 1]; ++b) {
+    const int32_t u = indices[b];
+    if (u >= v) continue;  // every edge once, from its larger end
+    int32_t ur = cc_find(parent, u);
+    while (vr != ur) {
+      if (vr < ur) {
+        const int32_t r = atomicCAS(parent + ur, ur, vr);
+        if (r == ur) break;
+        ur = r;  // ur was hooked meanwhile: continue from its new parent
+      } else {
+        const int32_t r = atomicCAS(parent + vr, vr, ur);
+        if (r == vr) {
+          vr = ur;
+          break;
+        }
+        vr = r;
+      }
+    }
   }
 }
 
-__global__ void k_rcm_start(int32_t N, const int64_t *__restrict__ seeds, const unsigned long long *__restrict__ best,
-                            int32_t *__restrict__ ord, int32_t *__restrict__ pos) {
-  const int32_t v = (int32_t)seeds[*best];
-  ord[N] = v;
-  pos[v] = N;
+// label[v] = root of v (in place over parent: writing a root only shortens paths)
+__global__ void k_cc_flatten(int64_t n, int32_t *parent) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int32_t r = (int32_t)v, p;
+  while ((p = cc_load(parent + r)) != r) r = p;
+  cc_store(parent + v, r);
 }
 
-__global__ void k_rcm_finish(int64_t n, const int32_t *__restrict__ ord, int32_t reverse, int32_t *__restrict__ order,
-                             int64_t *__restrict__ undo) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int32_t v = reverse ? ord[n - 1 - p] : ord[p];
+// rank[seeds[k]] = k; minr[root] = smallest rank in the component (minr starts at INT_MAX)
+__global__ void k_cc_rank(int64_t n, const int64_t *__restrict__ seeds, int32_t *__restrict__ rank,
+                          int32_t *__restrict__ minr) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) {
+    rank[seeds[k]] = (int32_t)k;
+    minr[k] = INT_MAX;
+  }
+}
+
+__global__ void k_cc_minrank(int64_t n, const int32_t *__restrict__ label, const int32_t *__restrict__ rank,
+                             int32_t *__restrict__ minr) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v < n) atomicMin(&minr[label[v]], rank[v]);
+}
+
+// flag[k] = 1 when seeds[k] is the first node of its component in the seed order
+__global__ void k_cc_flag(int64_t n, const int64_t *__restrict__ seeds, const int32_t *__restrict__ label,
+                          const int32_t *__restrict__ minr, int32_t *__restrict__ flag) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) flag[k] = minr[label[seeds[k]]] == (int32_t)k ? 1 : 0;
+}
+
+// component c (in seed order) starts from S[c]; croot[root] = c; csize zeroed
+__global__ void k_cc_seeds(int64_t n, const int64_t *__restrict__ seeds, const int32_t *__restrict__ label,
+                           const int32_t *__restrict__ flag, const int32_t *__restrict__ cidx,
+                           int32_t *__restrict__ S, int32_t *__restrict__ croot, int32_t *__restrict__ csize) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  csize[k] = 0;
+  if (flag[k]) {
+    const int32_t v = (int32_t)seeds[k];
+    S[cidx[k]] = v;
+    croot[label[v]] = cidx[k];
+  }
+}
+
+__global__ void k_cc_vertex(int64_t n, const int32_t *__restrict__ label, const int32_t *__restrict__ croot,
+                            int32_t *__restrict__ cv, int32_t *__restrict__ csize) {
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int32_t c = croot[label[v]];
+  cv[v] = c;
+  atomicAdd(&csize[c], 1);
+}
+
+// round 0: the seeds, one per component, each first in its component
+__global__ void k_rcm_seed_level(int32_t C, const int32_t *__restrict__ S, const int32_t *__restrict__ cstart,
+                                 int32_t *__restrict__ ord, int32_t *__restrict__ pos, int32_t *__restrict__ fin,
+                                 int32_t *__restrict__ done, int32_t *__restrict__ first,
+                                 int32_t *__restrict__ lcnt) {
+  const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  ord[c] = S[c];
+  pos[S[c]] = c;
+  fin[c] = cstart[c];
+  done[c] = 1;
+  first[c] = INT_MAX;
+  lcnt[c] = 0;
+}
+
+// final positions of a new level [e, e + L1): its nodes of component c form one segment,
+// whose first node follows the done[c] nodes of c's earlier levels
+__global__ void k_rcm_seg_first(int32_t e, int32_t L1, const int32_t *__restrict__ ord, const int32_t *__restrict__ cv,
+                                int32_t *__restrict__ first, int32_t *__restrict__ lcnt) {
+  const int32_t q = e + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= e + L1) return;
+  const int32_t c = cv[ord[q]];
+  atomicMin(&first[c], q);
+  atomicAdd(&lcnt[c], 1);
+}
+
+__global__ void k_rcm_seg_fin(int32_t e, int32_t L1, const int32_t *__restrict__ ord, const int32_t *__restrict__ cv,
+                              const int32_t *__restrict__ cstart, const int32_t *__restrict__ done,
+                              const int32_t *__restrict__ first, int32_t *__restrict__ fin) {
+  const int32_t q = e + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= e + L1) return;
+  const int32_t c = cv[ord[q]];
+  fin[q] = cstart[c] + done[c] + (q - first[c]);
+}
+
+__global__ void k_rcm_seg_close(int32_t e, int32_t L1, const int32_t *__restrict__ ord, const int32_t *__restrict__ cv,
+                                int32_t *__restrict__ first, int32_t *__restrict__ lcnt, int32_t *__restrict__ done) {
+  const int32_t q = e + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= e + L1) return;
+  const int32_t c = cv[ord[q]];
+  if (first[c] == q) {
+    done[c] += lcnt[c];
+    first[c] = INT_MAX;
+    lcnt[c] = 0;
+  }
+}
+
+__global__ void k_rcm_finish(int64_t n, const int32_t *__restrict__ ord, const int32_t *__restrict__ fin,
+                             int32_t reverse, int32_t *__restrict__ order, int64_t *__restrict__ undo) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int64_t p = reverse ? n - 1 - fin[q] : fin[q];
+  const int32_t v = ord[q];
   order[p] = v;
   undo[v] = p;
 }
-
-__global__ void k_rcm_best_init(unsigned long long *best) { *best = ULLONG_MAX; }
 
 // Input validation before any indexed access: seeds in [0, n), indptr non-decreasing from 0,
 // columns in [0, n). Sets *bad to 1 on a violation.
@@ -268,7 +363,11 @@ unsigned blocks_for(int64_t threads, int bs) { return (unsigned)((threads + bs -
 
 extern "C" {
 
-int64_t inch_rcm_device_work_size(int64_t n) { return 6 * (n + 2) * (int64_t)sizeof(int32_t) + 64; }
+constexpr int kRcmArrays = 20;  // int32 arrays of n + 2 entries in the work buffer
+
+int64_t inch_rcm_device_work_size(int64_t n) {
+  return (kRcmArrays * (n + 2) + 2 * (scan_tiles(n + 1) + 2)) * (int64_t)sizeof(int32_t) + 64;
+}
 
 inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *indices, const int64_t *seeds,
                             int32_t reverse, int32_t *order, int64_t *undo, void *work, int64_t work_bytes,
@@ -279,13 +378,13 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
   INCH_REQUIRE(work_bytes >= inch_rcm_device_work_size(n), "inch_rcm_device: work too small (%lld < %lld)",
                (long long)work_bytes, (long long)inch_rcm_device_work_size(n));
   hipStream_t st = (hipStream_t)stream;
-  unsigned long long *best = (unsigned long long *)work;          // first 64 bytes: scalars
-  int32_t *pos = (int32_t *)((char *)work + 64);
-  int32_t *claim = pos + (n + 2);
-  int32_t *cnt = claim + (n + 2);
-  int32_t *off = cnt + (n + 2);
-  int32_t *fill = off + (n + 2);
-  int32_t *ord = fill + (n + 2);
+  int32_t *arr[kRcmArrays];
+  for (int q = 0; q < kRcmArrays; ++q) arr[q] = (int32_t *)((char *)work + 64) + q * (n + 2);
+  int32_t *pos = arr[0], *claim = arr[1], *cnt = arr[2], *off = arr[3], *fill = arr[4], *ord = arr[5];
+  int32_t *label = arr[6], *rank = arr[7], *minr = arr[8], *flag = arr[9], *cidx = arr[10], *croot = arr[11];
+  int32_t *cv = arr[12], *csize = arr[13], *cstart = arr[14], *fin = arr[15], *first = arr[16], *lcnt = arr[17];
+  int32_t *done = arr[18], *S = arr[19];
+  int32_t *tsum = arr[kRcmArrays - 1] + (n + 2), *toff = tsum + scan_tiles(n + 1) + 2;
   const int bs = 256;
   int32_t *bad = (int32_t *)((char *)work + 8);
   k_rcm_flag_init<<<1, 1, 0, st>>>(bad);
@@ -306,53 +405,65 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
   HIP_TRY(hipStreamSynchronize(st));
   INCH_REQUIRE(!bad_h, "inch_rcm_device: seeds is not a permutation");
   k_rcm_init<<<blocks_for(n, bs), bs, 0, st>>>(n, pos, claim);
+  // components, their seeds (first node in the seed order) and sizes, in seed order
+  k_cc_init<<<blocks_for(n, bs), bs, 0, st>>>(n, label);
+  k_cc_hook<<<blocks_for(n, bs), bs, 0, st>>>(n, indptr, indices, label);
+  k_cc_flatten<<<blocks_for(n, bs), bs, 0, st>>>(n, label);
+  k_cc_rank<<<blocks_for(n, bs), bs, 0, st>>>(n, seeds, rank, minr);
+  k_cc_minrank<<<blocks_for(n, bs), bs, 0, st>>>(n, label, rank, minr);
+  k_cc_flag<<<blocks_for(n, bs), bs, 0, st>>>(n, seeds, label, minr, flag);
+  HIP_TRY(scan_exclusive_into<int32_t>(n, flag, cidx, tsum, toff, st));
+  k_cc_seeds<<<blocks_for(n, bs), bs, 0, st>>>(n, seeds, label, flag, cidx, S, croot, csize);
+  k_cc_vertex<<<blocks_for(n, bs), bs, 0, st>>>(n, label, croot, cv, csize);
   HIP_TRY(hipGetLastError());
-  int64_t N = 0, z = 0;
-  std::vector<int64_t> starts;   // BFS position of each component's seed
-  while (N < n) {
-    if (z >= n) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: seeds is not a permutation");
-    starts.push_back(N);
-    k_rcm_best_init<<<1, 1, 0, st>>>(best);
-    k_rcm_next_seed<<<blocks_for(n - z, bs), bs, 0, st>>>(n, z, seeds, pos, best);
-    unsigned long long kbest = 0;
-    HIP_TRY(hipMemcpyAsync(&kbest, best, sizeof(kbest), hipMemcpyDeviceToHost, st));
+  int32_t C = 0;
+  HIP_TRY(hipMemcpyAsync(&C, cidx + n, sizeof(C), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (C < 1 || C > n) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: %d components", C);
+  HIP_TRY(scan_exclusive_into<int32_t>(C, csize, cstart, tsum, toff, st));
+  k_rcm_seed_level<<<blocks_for(C, bs), bs, 0, st>>>(C, S, cstart, ord, pos, fin, done, first, lcnt);
+  HIP_TRY(hipGetLastError());
+  // all components' levels advance together
+  int32_t s = 0, e = C;
+  int64_t N = C;
+  while (true) {
+    const int32_t L = e - s;
+    k_rcm_zero<<<blocks_for(L, bs), bs, 0, st>>>(L, cnt, fill);
+    k_rcm_claim<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim);
+    k_rcm_count<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim, cnt);
+    HIP_TRY(scan_exclusive_into<int32_t>(L, cnt, off, tsum, toff, st));
+    int32_t L1 = 0;
+    HIP_TRY(hipMemcpyAsync(&L1, off + L, sizeof(L1), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    if (kbest == ULLONG_MAX) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: seeds is not a permutation");
-    z = (int64_t)kbest + 1;
-    k_rcm_start<<<1, 1, 0, st>>>((int32_t)N, seeds, best, ord, pos);
-    int32_t s = (int32_t)N, e = (int32_t)N + 1;
-    N += 1;
-    while (true) {
-      const int32_t L = e - s;
-      k_rcm_zero<<<blocks_for(L, bs), bs, 0, st>>>(L, cnt, fill);
-      k_rcm_claim<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim);
-      k_rcm_count<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim, cnt);
-      k_rcm_scan<<<1, 1024, 0, st>>>(L, cnt, off);
-      int32_t L1 = 0;
-      HIP_TRY(hipMemcpyAsync(&L1, off + L, sizeof(L1), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      if (L1 < 0 || N + L1 > n) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: inconsistent graph (level %d)", L1);
-      if (L1 == 0) break;
-      k_rcm_scatter<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim, off,
-                                                                    fill);
-      k_rcm_sort_groups<<<(unsigned)L, 64, 0, st>>>(L, e, ord, indptr, off);
-      k_rcm_place<<<blocks_for(L1, bs), bs, 0, st>>>(e, L1, ord, pos);
-      HIP_TRY(hipGetLastError());
-      s = e;
-      e = e + L1;
-      N += L1;
-    }
+    if (L1 < 0 || N + L1 > n) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: inconsistent graph (level %d)", L1);
+    if (L1 == 0) break;
+    k_rcm_scatter<<<blocks_for((int64_t)L * kG, bs), bs, 0, st>>>(s, e, ord, indptr, indices, pos, claim, off,
+                                                                  fill);
+    k_rcm_sort_groups<<<(unsigned)L, 64, 0, st>>>(L, e, ord, indptr, off);
+    k_rcm_place<<<blocks_for(L1, bs), bs, 0, st>>>(e, L1, ord, pos);
+    k_rcm_seg_first<<<blocks_for(L1, bs), bs, 0, st>>>(e, L1, ord, cv, first, lcnt);
+    k_rcm_seg_fin<<<blocks_for(L1, bs), bs, 0, st>>>(e, L1, ord, cv, cstart, done, first, fin);
+    k_rcm_seg_close<<<blocks_for(L1, bs), bs, 0, st>>>(e, L1, ord, cv, first, lcnt, done);
+    HIP_TRY(hipGetLastError());
+    s = e;
+    e = e + L1;
+    N += L1;
   }
-  k_rcm_finish<<<blocks_for(n, bs), bs, 0, st>>>(n, ord, reverse, order, undo);
+  if (N != n) INCH_FAIL(INCH_ERR_INVALID, "inch_rcm_device: %lld of %lld nodes ordered", (long long)N, (long long)n);
+  k_rcm_finish<<<blocks_for(n, bs), bs, 0, st>>>(n, ord, fin, reverse, order, undo);
   HIP_TRY(hipGetLastError());
+  std::vector<int32_t> cs;
+  if (comp_ptr) {
+    cs.resize((size_t)C + 1);
+    HIP_TRY(hipMemcpyAsync(cs.data(), cstart, (C + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  }
   HIP_TRY(hipStreamSynchronize(st));
   // components are contiguous in the output order: their start offsets, ascending, then n
-  const int64_t C = (int64_t)starts.size();
   if (n_comp) *n_comp = C;
   if (comp_ptr) {
-    if (C + 1 > comp_cap) INCH_FAIL(INCH_ERR_CAPACITY, "inch_rcm_device: %lld components exceed comp_cap", (long long)C);
+    if (C + 1 > comp_cap) INCH_FAIL(INCH_ERR_CAPACITY, "inch_rcm_device: %d components exceed comp_cap", C);
     for (int64_t c = 0; c < C; ++c) {
-      const int64_t s0 = starts[c], s1 = c + 1 < C ? starts[c + 1] : n;
+      const int64_t s0 = cs[c], s1 = cs[c + 1];
       comp_ptr[reverse ? C - 1 - c : c] = reverse ? n - s1 : s0;
     }
     comp_ptr[C] = n;

@@ -78,17 +78,15 @@ def rcm_host(indptr, indices, seed_order=None, reverse=True):
 def rcm_device(indptr, indices, seed_order=None, reverse=True, stream=None, components=False):
     """3DRCM on the device: (order int32 [n], undo int64 [n]) device tensors, bit-exact to
     rcm_host. indptr int64 / indices int32 device tensors (unique columns per row).
-    seed_order: optional seed order (default: stable argsort of the degree, computed on the
-    host from the row pointers with a radix sort). ``components``: also return the start
-    offsets of the connected components in the output order (numpy int64 [C + 1])."""
+    seed_order: optional seed order (default: stable argsort of the degree, ties in index
+    order as numpy's stable argsort, sorted on the device). ``components``: also return the
+    start offsets of the connected components in the output order (numpy int64 [C + 1])."""
     n = indptr.numel() - 1
     dev = indptr.device
     if n <= 0:
         raise ValueError("empty graph")
     if seed_order is None:
-        deg = np.diff(indptr.cpu().numpy())
-        key = deg.astype(np.int16) if deg.max(initial=0) < 32767 else deg
-        seeds = torch.as_tensor(np.argsort(key, kind="stable").astype(np.int64), device=dev)
+        seeds = torch.argsort(indptr[1:] - indptr[:-1], stable=True)
     else:
         seeds = torch.as_tensor(np.ascontiguousarray(seed_order, dtype=np.int64), device=dev).reshape(-1)
         if seeds.numel() != n:

@@ -69,17 +69,22 @@ def test_rcm_pipeline_matches_oracle(glob2k):
     assert np.array_equal(o, o_ref) and np.array_equal(u, u_ref)
 
 
-@pytest.mark.parametrize("case", ["globule", "blobs_and_singletons", "numpy_default_seeds", "forward", "40k"])
+@pytest.mark.parametrize("case", ["globule", "blobs_and_singletons", "blobs_random_seeds", "many_blobs",
+                                  "numpy_default_seeds", "forward", "40k"])
 def test_rcm_device_bit_exact(glob2k, glob40k, case):
     """The level-synchronous device RCM gives the oracle's permutation (and, at 40k atoms,
     the sequential host implementation's), for several components and isolated atoms, an
-    explicit seed order and the unreversed order."""
+    explicit seed order (also a random one, which interleaves the components' first seeds),
+    40 components of different depths advancing together, and the unreversed order."""
     from inching_amd import ops
     _print("case: device rcm %s" % case)
     rng = np.random.default_rng(5)
-    if case == "blobs_and_singletons":
+    if case in ("blobs_and_singletons", "blobs_random_seeds"):
         X = np.concatenate([rng.uniform(0, 15, (300, 3)), rng.uniform(60, 70, (150, 3)),
                             np.array([[200.0, 0, 0], [-200.0, 0, 0], [0, 300.0, 0]])])
+    elif case == "many_blobs":
+        X = np.concatenate([rng.uniform(0, 6 + 2 * (b % 7), (40 + 13 * b, 3)) + np.array([100.0 * b, 0, 0])
+                            for b in range(40)])
     elif case == "40k":
         X = glob40k
     else:
@@ -88,6 +93,8 @@ def test_rcm_device_bit_exact(glob2k, glob40k, case):
     ip_d, ix_d = ops.neighbours(xyz, 8.0)
     ip, ix = ip_d.cpu().numpy(), ix_d.cpu().numpy()
     seeds = np.argsort(np.diff(ip)) if case == "numpy_default_seeds" else None
+    if case == "blobs_random_seeds":
+        seeds = rng.permutation(len(ip) - 1)
     reverse = case != "forward"
     o, u, comps = ops.rcm_device(ip_d, ix_d, seed_order=seeds, reverse=reverse, components=True)
     o, u = o.cpu().numpy(), u.cpu().numpy()
@@ -98,8 +105,10 @@ def test_rcm_device_bit_exact(glob2k, glob40k, case):
     lab_o = lab[o]
     want = np.r_[np.nonzero(np.r_[True, lab_o[1:] != lab_o[:-1]])[0], len(o)]
     assert len(want) == nc + 1 and np.array_equal(comps, want)
-    if case == "blobs_and_singletons":
+    if case in ("blobs_and_singletons", "blobs_random_seeds"):
         assert nc == 5
+    if case == "many_blobs":
+        assert nc == 40
     if case == "40k":
         o_ref, u_ref = ops.rcm_host(ip, ix)
     else:
