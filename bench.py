@@ -46,8 +46,9 @@ def parse():
     ap.add_argument("--atoms", type=int, default=300000)
     ap.add_argument("--k", type=int, default=64)
     ap.add_argument("--mode", default="mf64", help="headline operator mode: mf64 | mixed | f64")
-    ap.add_argument("--compare", default="f64,mixed,warm",
-                    help="other modes timed once each (warm: the headline operator from the learned start)")
+    ap.add_argument("--compare", default="f64,mixed,warm,mf64_rows",
+                    help="other modes timed once each (warm: the headline operator from the learned start; "
+                         "mf64_rows: the headline operator with its recurrence vectors in the matrix's order)")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-atoms", type=int, default=20000, help="atoms of the CPU baseline sample")
@@ -215,7 +216,7 @@ def _roofline(A, kind, epi, ms_total, cnt, step_s, workload):
            "kernel": key + " (" + KERNEL_DESC[kind] + ")", "epilogue": epi, "avg_launch_ms": round(avg_ms, 5),
            "launches": cnt, "bytes_per_launch": int(by),
            "kernel_share_of_step": round(ms_total / 1e3 / step_s, 3) if step_s else None}
-    if kind == "mf64":
+    if kind in ("mf64", "mf64rec"):
         tf = 18.0 * A.nnzb / (avg_ms * 1e-3) / 1e12
         out["useful_fp64"] = {"achieved_tflops": round(tf, 2), "peak_tflops": FP64_VECTOR_PEAK_TFLOPS,
                               "frac": round(tf / FP64_VECTOR_PEAK_TFLOPS, 4),
@@ -293,6 +294,14 @@ def run_assembly(args):
             if net is None:
                 net = W.WarmStartNet.load()
             v0 = lambda A_, seg: W.warm_start_vector(net, A_, seg)          # noqa: E731
+        if mode == "mf64_rows":             # matrix-free float64, recurrence vectors in the matrix's order
+            from inching_amd import operator as OP
+            OP.RECORDS = False
+            try:
+                return solve_assembly(xyz, K, precision="mf64", tol_rel=args.tol, timer=timer, seed=0,
+                                      verbose=args.verbose)
+            finally:
+                OP.RECORDS = True
         return solve_assembly(xyz, K, precision="mf64" if mode == "warm" else mode, tol_rel=args.tol,
                               timer=timer, seed=0, verbose=args.verbose, v0=v0)
 
@@ -360,6 +369,8 @@ def run_assembly(args):
             rf["frac_of_stream_copy"] = round(rf["achieved"] / copy_gbs, 4)
     if "f64" in compare:
         summary["speedup_vs_plain_f64"] = round(compare["f64"]["seconds"] / sec, 3)
+    if "mf64_rows" in compare:
+        summary["speedup_records_vs_matrix_order"] = round(compare["mf64_rows"]["seconds"] / sec, 3)
     for mode in ("mixed", "warm"):
         if mode in compare and "f64" in compare:
             summary["speedup_%s_vs_plain_f64" % mode] = round(compare["f64"]["seconds"] / compare[mode]["seconds"], 3)
@@ -496,6 +507,13 @@ def kernel_bytes(kind, A, epi):
     from inching_amd.autotune import best_config
     nnzb, n_atoms = A.nnzb, A.n_atoms
     nd = 3 * n_atoms
+    if kind == "mf64rec":
+        # record form: lists (slot words) + cluster pointers, every slot's record (coordinates
+        # and x, 48 B) read, and per slot 24 B of z read, 24 B of output written, 48 B of the
+        # accumulator read and written
+        _, ptr, _, ent = A.clusters()
+        slots = 4 * (ptr.numel() - 1)
+        return 4 * ent.numel() + 8 * ptr.numel() + (144 if epi == "cheb" else 72) * slots
     if kind == "f32":
         s, sv = 4, 4 if epi == "cheb" else 8
         return 9 * s * nnzb + (5 if epi == "cheb" else 2) * nd * sv
@@ -512,12 +530,15 @@ def kernel_bytes(kind, A, epi):
 
 KERNEL_DESC = {"f32": "block SpMV, fp32 values, Chebyshev epilogue",
                "f64": "block SpMV, fp64 values, Chebyshev epilogue",
-               "mf64": "matrix-free fp64 ANM product, Chebyshev epilogue"}
+               "mf64": "matrix-free fp64 ANM product, Chebyshev epilogue",
+               "mf64rec": "matrix-free fp64 ANM product on cluster-order records, Chebyshev epilogue"}
 
 
 def kernel_key(kind, A):
     """Template name of the operator kernel the autotuner picked for this matrix."""
     from inching_amd.autotune import best_config
+    if kind == "mf64rec":
+        return "k_matfree_rec<true>"
     cfg = best_config(A, kind)
     if kind == "mf64":
         c, f = cfg % 1000, cfg // 1000

@@ -17,6 +17,14 @@ import torch
 from . import ops
 
 BSR_LANES = (4, 8, 16, 32, 64)
+# Times within NOISE of the fastest count as ties, resolved by a fixed preference order
+# (candidates listed first win), so that the choice does not flip between runs on timing
+# noise: block SpMV shapes with 16-bit columns and XCD-contiguous rows first (the lowest
+# measured HBM traffic), wide row groups before narrow ones; cluster kernels before per-row.
+NOISE = 0.02
+BSR_PREFERENCE = tuple(1200 + g for g in (32, 64, 16, 8, 4)) + tuple(200 + g for g in (32, 64, 16, 8, 4)) + \
+    (32, 64, 16, 8, 4)
+CLUSTER_PREFERENCE = (6001, 6104, 6002, 6108, 6004, 6116, 4001, 4104, 4002, 4108, 4004, 4116)
 MATFREE_CONFIGS = (41, 42, 81, 82, 161, 162, 321, 322)
 CLUSTER_CONFIGS = (1, 2, 4, 104, 108, 116)   # list entries per lane iteration; 100 + clusters per wave (persistent)
 
@@ -93,7 +101,9 @@ def best_config(A, precision, reps=5):
         for cfg in codes:
             times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
                                reps)
+        order = CLUSTER_PREFERENCE + tuple(c for c in codes if c < 4000)
     else:
+        order = BSR_PREFERENCE
         # int32 columns: code = lanes; 16-bit column offsets: code = 200 + lanes
         vals = A.vals(precision)
         for lanes in BSR_LANES:
@@ -106,10 +116,19 @@ def best_config(A, precision, reps=5):
                     times[base + lanes] = _time(lambda: ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, s1=0.1,
                                                                      c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, lanes=arg,
                                                                      col16=c16), reps)
-    best = min(times, key=times.get)
+    best = _pick(times, order)
     _cache[k] = {"best": int(best), "ms": {str(c): round(t, 5) for c, t in times.items()}}
     _save()
     return int(best)
+
+
+def _pick(times, order):
+    """The first candidate in ``order`` whose time is within NOISE of the fastest."""
+    tmin = min(times.values())
+    for c in order:
+        if c in times and times[c] <= tmin * (1.0 + NOISE):
+            return c
+    return min(times, key=times.get)
 
 
 def bsr_launch(A, code):

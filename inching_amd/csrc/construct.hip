@@ -230,76 +230,31 @@ __global__ void k_image_masks(int64_t n, const double *__restrict__ xyz, double 
   for (int64_t e = indptr[i]; e < indptr[i + 1]; ++e) images[e] = (uint16_t)image_mask(xyz, i, indices[e], bx, by, rc2);
 }
 
-// One block (256 threads) per row: bitonic sort of the row's columns in LDS.
-__global__ void __launch_bounds__(256) k_sort_rows(int64_t n, const int64_t *__restrict__ indptr,
-                                                    int32_t *__restrict__ indices, int32_t *__restrict__ overflow) {
-  __shared__ int32_t buf[kMaxRow];
-  int64_t row = blockIdx.x;
-  if (row >= n) return;
-  int64_t b = indptr[row], e = indptr[row + 1];
-  int len = (int)(e - b);
-  if (len > kMaxRow) {
-    if (threadIdx.x == 0) atomicMax(overflow, len);
-    return;
-  }
-  int p2 = 1;
-  while (p2 < len) p2 <<= 1;
-  for (int t = threadIdx.x; t < p2; t += blockDim.x) buf[t] = t < len ? indices[b + t] : 0x7fffffff;
-  __syncthreads();
-  for (int k = 2; k <= p2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < p2; t += blockDim.x) {
-        int ixj = t ^ j;
-        if (ixj > t) {
-          bool up = (t & k) == 0;
-          int32_t a = buf[t], c = buf[ixj];
-          if ((a > c) == up) {
-            buf[t] = c;
-            buf[ixj] = a;
-          }
-        }
+// Sorted copy of a row held in LDS (buf[0..len)) into out[0..len), ascending, by one block
+// (the caller has synchronised after filling buf). Rows of up to 256 entries by rank (each
+// entry's place = the number of smaller entries, ties by position); longer ones by a
+// bitonic sort in place over the next power of two (buf holds kMaxRow entries).
+__device__ void sort_row_to(int32_t *buf, int len, int32_t *__restrict__ out) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  if (len <= 256) {
+    for (int t = tid; t < len; t += nt) {
+      const int32_t v = buf[t];
+      int r = 0;
+      for (int q = 0; q < len; ++q) {
+        const int32_t w = buf[q];
+        r += (w < v) | ((w == v) & (q < t));
       }
-      __syncthreads();
+      out[r] = v;
     }
-  }
-  for (int t = threadIdx.x; t < len; t += blockDim.x) indices[b + t] = buf[t];
-}
-
-// counts[i] = length of row order[i] of the source CSR
-__global__ void k_perm_counts(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ order,
-                              int64_t *__restrict__ counts) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t o = order[i];
-  counts[i] = indptr[o + 1] - indptr[o];
-}
-
-// One block per new row: the columns of source row order[row], renumbered by undo (old index
-// -> new index), sorted ascending by a bitonic sort in LDS.
-__global__ void __launch_bounds__(256) k_permute_sort_rows(int64_t n, const int64_t *__restrict__ indptr,
-                                                            const int32_t *__restrict__ indices,
-                                                            const int32_t *__restrict__ order,
-                                                            const int64_t *__restrict__ undo,
-                                                            const int64_t *__restrict__ out_ptr,
-                                                            int32_t *__restrict__ out_idx,
-                                                            int32_t *__restrict__ overflow) {
-  __shared__ int32_t buf[kMaxRow];
-  const int64_t row = blockIdx.x;
-  if (row >= n) return;
-  const int64_t o = order[row];
-  const int64_t b = indptr[o];
-  const int len = (int)(indptr[o + 1] - b);
-  if (len > kMaxRow) {
-    if (threadIdx.x == 0) atomicMax(overflow, len);
     return;
   }
   int p2 = 1;
   while (p2 < len) p2 <<= 1;
-  for (int t = threadIdx.x; t < p2; t += blockDim.x) buf[t] = t < len ? (int32_t)undo[indices[b + t]] : 0x7fffffff;
+  for (int t = len + tid; t < p2; t += nt) buf[t] = 0x7fffffff;
   __syncthreads();
   for (int k = 2; k <= p2; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = threadIdx.x; t < p2; t += blockDim.x) {
+      for (int t = tid; t < p2; t += nt) {
         const int ixj = t ^ j;
         if (ixj > t) {
           const bool up = (t & k) == 0;
@@ -313,8 +268,57 @@ __global__ void __launch_bounds__(256) k_permute_sort_rows(int64_t n, const int6
       __syncthreads();
     }
   }
-  const int64_t ob = out_ptr[row];
-  for (int t = threadIdx.x; t < len; t += blockDim.x) out_idx[ob + t] = buf[t];
+  for (int t = tid; t < len; t += nt) out[t] = buf[t];
+}
+
+// One 64-thread block per row: the row's columns sorted ascending in place.
+__global__ void __launch_bounds__(64) k_sort_rows(int64_t n, const int64_t *__restrict__ indptr,
+                                                   int32_t *__restrict__ indices, int32_t *__restrict__ overflow) {
+  __shared__ int32_t buf[kMaxRow];
+  const int64_t row = blockIdx.x;
+  if (row >= n) return;
+  const int64_t b = indptr[row], e = indptr[row + 1];
+  const int len = (int)(e - b);
+  if (len > kMaxRow) {
+    if (threadIdx.x == 0) atomicMax(overflow, len);
+    return;
+  }
+  for (int t = threadIdx.x; t < len; t += blockDim.x) buf[t] = indices[b + t];
+  __syncthreads();
+  sort_row_to(buf, len, indices + b);
+}
+
+// counts[i] = length of row order[i] of the source CSR
+__global__ void k_perm_counts(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ order,
+                              int64_t *__restrict__ counts) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t o = order[i];
+  counts[i] = indptr[o + 1] - indptr[o];
+}
+
+// One block per new row: the columns of source row order[row], renumbered by undo (old index
+// -> new index), sorted ascending (sort_row_to).
+__global__ void __launch_bounds__(64) k_permute_sort_rows(int64_t n, const int64_t *__restrict__ indptr,
+                                                           const int32_t *__restrict__ indices,
+                                                           const int32_t *__restrict__ order,
+                                                           const int64_t *__restrict__ undo,
+                                                           const int64_t *__restrict__ out_ptr,
+                                                           int32_t *__restrict__ out_idx,
+                                                           int32_t *__restrict__ overflow) {
+  __shared__ int32_t buf[kMaxRow];
+  const int64_t row = blockIdx.x;
+  if (row >= n) return;
+  const int64_t o = order[row];
+  const int64_t b = indptr[o];
+  const int len = (int)(indptr[o + 1] - b);
+  if (len > kMaxRow) {
+    if (threadIdx.x == 0) atomicMax(overflow, len);
+    return;
+  }
+  for (int t = threadIdx.x; t < len; t += blockDim.x) buf[t] = (int32_t)undo[indices[b + t]];
+  __syncthreads();
+  sort_row_to(buf, len, out_idx + out_ptr[row]);
 }
 
 // One 64-lane wave per row: H_ij = -d d^T / r^2 for j != i (r^2 > 0); the diagonal
@@ -443,13 +447,22 @@ __global__ void k_cl_atoms(int64_t n, int64_t slots, const int32_t *__restrict__
   if (q < slots) cluster_atoms[q] = q < n ? atoms[q] : -1;
 }
 
+// Squared distance of atoms i and j as the cluster product computes it (d = p_j - p_i, the
+// same FMAs): a pair at r2 = 0 (coincident atoms, whose term the reference masks) gets no
+// list bit, so the product needs no test of its own.
+__device__ __forceinline__ double cl_pair_r2(const double *__restrict__ xyz, int64_t i, int64_t j) {
+  const double d0 = xyz[3 * j] - xyz[3 * i], d1 = xyz[3 * j + 1] - xyz[3 * i + 1], d2 = xyz[3 * j + 2] - xyz[3 * i + 2];
+  return __builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0));
+}
+
 // One block per cluster. The rows of its atoms are gathered as keys (j << 3 | t), sorted
 // (bitonic, LDS), and each run of equal j becomes one entry with the OR of its atom bits.
 // Count pass (FILL = false): counts[c] = entries, base[c] = smallest j; overflow bit 1 =
 // more than kMaxUnion row entries, bit 2 = the union spans 2^24 columns or more.
 // Fill pass: entries[cl_ptr[c] + r] = ((j - base[c]) << 8) | mask, ascending j.
 template <bool FILL>
-__global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const int64_t *__restrict__ rowptr,
+__global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const double *__restrict__ xyz,
+                                                   const int64_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col,
                                                    const int32_t *__restrict__ cluster_atoms,
                                                    const int64_t *__restrict__ cl_ptr, int32_t *__restrict__ base,
@@ -526,7 +539,7 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const int64_t *__
     if (e == 0 || (uint32_t)(key[e - 1] >> 3) != j) {
       for (int f = e; f < total && (uint32_t)(key[f] >> 3) == j; ++f) {
         const int t = (int)(key[f] & 7);
-        if ((uint32_t)ca[t] != j) m |= 1 << t;
+        if ((uint32_t)ca[t] != j && cl_pair_r2(xyz, ca[t], j) > 0.0) m |= 1 << t;
       }
     }
     val[e] = m;
@@ -585,6 +598,63 @@ __global__ void k_cl_permute(int64_t ncl, const uint64_t *__restrict__ perm, con
   for (int t = 0; t < kClusterAtoms; ++t) atoms_out[kClusterAtoms * c + t] = atoms_in[kClusterAtoms * src + t];
   counts_out[c] = counts_in[src];
   base_out[c] = base_in[src];
+}
+
+// ---- record form of the cluster layout (the operator's Chebyshev recurrence) ----
+// rec[6 q .. 6 q + 2] = coordinates of slot q = 4 c + t (atom cluster_atoms[q]; zeros for a
+// padding slot), rec[6 q + 3 .. 6 q + 5] = 0 (the vector part, written by the pack and the
+// product); slot_of[atom] = its slot.
+__global__ void k_rec_coords(int64_t slots, const int32_t *__restrict__ cluster_atoms, const double *__restrict__ xyz,
+                             double *__restrict__ rec, int32_t *__restrict__ slot_of) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= slots) return;
+  const int32_t at = cluster_atoms[q];
+  for (int k = 0; k < 3; ++k) {
+    rec[6 * q + k] = at >= 0 ? xyz[3 * (int64_t)at + k] : 0.0;
+    rec[6 * q + 3 + k] = 0.0;
+  }
+  if (at >= 0) slot_of[at] = (int32_t)q;
+}
+
+// One block per cluster: its list rewritten as (slot of j) << 4 | mask and sorted ascending
+// (bitonic sort in LDS; a list holds fewer than kMaxUnion entries).
+__global__ void __launch_bounds__(256) k_rec_lists(int64_t ncl, const int64_t *__restrict__ cl_ptr,
+                                                    const int32_t *__restrict__ base,
+                                                    const uint32_t *__restrict__ entries,
+                                                    const int32_t *__restrict__ slot_of, uint32_t *__restrict__ out) {
+  __shared__ uint32_t key[kMaxUnion];
+  const int64_t c = blockIdx.x;
+  const int64_t e0 = cl_ptr[c];
+  const int len = (int)(cl_ptr[c + 1] - e0);
+  const uint32_t b0 = (uint32_t)base[c];
+  int p2 = 1;
+  while (p2 < len) p2 <<= 1;
+  for (int e = threadIdx.x; e < p2; e += blockDim.x) {
+    uint32_t k = 0xffffffffu;
+    if (e < len) {
+      const uint32_t w = entries[e0 + e];
+      k = ((uint32_t)slot_of[b0 + (w >> 8)] << 4) | (w & 15u);
+    }
+    key[e] = k;
+  }
+  __syncthreads();
+  for (int k = 2; k <= p2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int t = threadIdx.x; t < p2; t += blockDim.x) {
+        const int ixj = t ^ j;
+        if (ixj > t) {
+          const bool up = (t & k) == 0;
+          const uint32_t a = key[t], b = key[ixj];
+          if ((a > b) == up) {
+            key[t] = b;
+            key[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = threadIdx.x; e < len; e += blockDim.x) out[e0 + e] = key[e];
 }
 
 struct DevBuf {
@@ -697,7 +767,7 @@ extern "C" inch_status inch_neighbors_celllist(int64_t n, const double *xyz, dou
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMalloc(&ovf.p, sizeof(int32_t)));
   HIP_TRY(hipMemsetAsync(ovf.p, 0, sizeof(int32_t), st));
-  k_sort_rows<<<n, 256, 0, st>>>(n, indptr, indices, (int32_t *)ovf.p);
+  k_sort_rows<<<n, 64, 0, st>>>(n, indptr, indices, (int32_t *)ovf.p);
   HIP_TRY(hipGetLastError());
   int32_t over = 0;
   HIP_TRY(hipMemcpyAsync(&over, ovf.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -720,7 +790,7 @@ extern "C" inch_status inch_csr_permute(int64_t n, const int64_t *indptr, const 
   k_perm_counts<<<(unsigned)((n + bs - 1) / bs), bs, 0, st>>>(n, indptr, order, (int64_t *)counts.p);
   HIP_TRY(hipGetLastError());
   SCAN_TRY(n, (const int64_t *)counts.p, out_indptr);
-  k_permute_sort_rows<<<(unsigned)n, 256, 0, st>>>(n, indptr, indices, order, undo, out_indptr, out_indices,
+  k_permute_sort_rows<<<(unsigned)n, 64, 0, st>>>(n, indptr, indices, order, undo, out_indptr, out_indices,
                                                   (int32_t *)ovf.p);
   HIP_TRY(hipGetLastError());
   int32_t over = 0;
@@ -820,7 +890,7 @@ extern "C" inch_status inch_neighbors_celllist_pbc(int64_t n, const double *xyz,
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMalloc(&ovf.p, sizeof(int32_t)));
   HIP_TRY(hipMemsetAsync(ovf.p, 0, sizeof(int32_t), st));
-  k_sort_rows<<<n, 256, 0, st>>>(n, indptr, indices, (int32_t *)ovf.p);
+  k_sort_rows<<<n, 64, 0, st>>>(n, indptr, indices, (int32_t *)ovf.p);
   HIP_TRY(hipGetLastError());
   int32_t over = 0;
   HIP_TRY(hipMemcpyAsync(&over, ovf.p, sizeof(int32_t), hipMemcpyDeviceToHost, st));
@@ -873,7 +943,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
     HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (total > capacity) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters: need %lld > capacity %lld", (long long)total, (long long)capacity);
-    k_cl_union<true><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, cluster_atoms, cl_ptr, base, nullptr, entries,
+    k_cl_union<true><<<(unsigned)ncl, 256, 0, st>>>(ncl, xyz, rowptr, col, cluster_atoms, cl_ptr, base, nullptr, entries,
                                                     (int32_t *)ovf.p);
     HIP_TRY(hipGetLastError());
     inch_status s = check_overflow("fill");
@@ -943,7 +1013,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   HIP_TRY(hipMalloc(&base0.p, ncl * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&perm.p, ncl * sizeof(uint64_t)));
   HIP_TRY(hipMemcpyAsync(atoms0.p, cluster_atoms, slots * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
+  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, xyz, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
                                                    (int64_t *)counts0.p, nullptr, (int32_t *)ovf.p);
   HIP_TRY(hipGetLastError());
   {
@@ -974,5 +1044,26 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (nnz_out) *nnz_out = total;
+  return INCH_OK;
+}
+
+extern "C" inch_status inch_anm_records(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
+                                        const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries,
+                                        double *rec, uint32_t *rec_entries, void *stream) {
+  INCH_REQUIRE(n > 0 && xyz && cluster_atoms && cl_ptr && base && entries && rec && rec_entries,
+               "inch_anm_records: bad args");
+  INCH_REQUIRE(ncl == (n + kClusterAtoms - 1) / kClusterAtoms, "inch_anm_records: ncl must be ceil(n / 4)");
+  // the record product addresses records with 32-bit byte offsets (48 bytes per slot)
+  if (192 * ncl > (int64_t)UINT32_MAX)
+    INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_records: %lld clusters exceed the 32-bit record offsets", (long long)ncl);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t slots = kClusterAtoms * ncl;
+  DevBuf slot_of;
+  HIP_TRY(hipMalloc(&slot_of.p, n * sizeof(int32_t)));
+  k_rec_coords<<<(unsigned)((slots + 255) / 256), 256, 0, st>>>(slots, cluster_atoms, xyz, rec, (int32_t *)slot_of.p);
+  HIP_TRY(hipGetLastError());
+  k_rec_lists<<<(unsigned)ncl, 256, 0, st>>>(ncl, cl_ptr, base, entries, (const int32_t *)slot_of.p, rec_entries);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));   // slot_of is freed on return
   return INCH_OK;
 }

@@ -160,21 +160,21 @@ __device__ __forceinline__ double recip_r2(double r2) {
 }
 
 // One neighbour term of a cluster atom: a += d (d . (vi - xj)) / r2 with d = pj - pi, for a
-// set list bit and r2 > 0 (coincident atoms are masked as in the reference). Every rounding
-// is fixed (explicit FMAs), so all forms of the cluster product add bitwise-equal terms.
-// The conditional becomes an exec-masked branch around the reciprocal and the dot, which a
-// wave skips when none of its lanes has the bit (about 6% of the atom blocks on a globule);
-// a straight-line form with selects measured 4% slower.
+// set list bit. The layout sets no bit for a pair at r2 = 0 (coincident atoms, masked as in
+// the reference), so a set bit implies r2 > 0. Every rounding is fixed (explicit FMAs), so
+// all forms of the cluster product add bitwise-equal terms. The whole term is one
+// exec-masked branch, which a wave skips when none of its lanes has the bit.
 __device__ __forceinline__ void cl_term(bool bit, const double (&pi)[3], const double (&vi)[3], double pj0,
                                         double pj1, double pj2, double xj0, double xj1, double xj2, double *a) {
-  const double d0 = pj0 - pi[0], d1 = pj1 - pi[1], d2 = pj2 - pi[2];
-  const double r2 = __builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0));
-  const double w0 = vi[0] - xj0, w1 = vi[1] - xj1, w2 = vi[2] - xj2;
-  const bool on = bit && r2 > 0.0;
-  const double s = on ? __builtin_fma(d2, w2, __builtin_fma(d1, w1, d0 * w0)) * recip_r2(r2) : 0.0;
-  a[0] = __builtin_fma(d0, s, a[0]);
-  a[1] = __builtin_fma(d1, s, a[1]);
-  a[2] = __builtin_fma(d2, s, a[2]);
+  if (bit) {
+    const double d0 = pj0 - pi[0], d1 = pj1 - pi[1], d2 = pj2 - pi[2];
+    const double r2 = __builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0));
+    const double w0 = vi[0] - xj0, w1 = vi[1] - xj1, w2 = vi[2] - xj2;
+    const double s = __builtin_fma(d2, w2, __builtin_fma(d1, w1, d0 * w0)) * recip_r2(r2);
+    a[0] = __builtin_fma(d0, s, a[0]);
+    a[1] = __builtin_fma(d1, s, a[1]);
+    a[2] = __builtin_fma(d2, s, a[2]);
+  }
 }
 
 // y_i = plus_i x_i + sum_{j != i, r2 > 0} d (d . (x_i - x_j)) / r2,  d = p_j - p_i
@@ -573,6 +573,97 @@ k_matfree_clp(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int3
     y[dof] = (VT)out;
     if (acc) acc[dof] = (VT)((double)acc[dof] + mu * out);
   }
+}
+
+// Record form of the cluster product, for the Chebyshev recurrence. The operator keeps its
+// recurrence vectors in records rec[6 q .. 6 q + 5] = (coordinates, vector entries) of slot
+// q = 4 c + t (atom t of cluster c; a padding slot holds zeros) and the lists in absolute
+// slot numbers, (q_j << 4) | mask, ascending q_j (inch_anm_records). A neighbour is then one
+// 48-byte record read by three 16-byte loads, and a neighbouring cluster's four atoms are
+// adjacent records: on the 2.4M-atom assembly this takes the product from 0.79 to 0.57 ms
+// (gathers from the coordinate and vector arrays in the matrix's order touch about twice
+// the cache lines). The epilogue is that of k_matfree_cl: rout's vector part = s1 (A x +
+// (plus_i - c0) x) - c2 z with x and z the vector parts of rin and rz, acc += mu rout.
+// rout's coordinate part is not written (the caller's buffers carry the coordinates).
+template <bool XCD>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+k_matfree_rec(int64_t ncl, const double *__restrict__ rin, const double *__restrict__ rz, double *__restrict__ rout,
+              const int64_t *__restrict__ cl_ptr, const uint32_t *__restrict__ entries, double plus_i, double s1,
+              double c0, double c2, double *__restrict__ acc, double mu) {
+  const int64_t c = xcd_block(XCD);
+  if (c >= ncl) return;
+  const int lane = threadIdx.x;
+  __shared__ double red[3][64];
+  double pi[kClAtoms][3], vi[kClAtoms][3];
+#pragma unroll
+  for (int t = 0; t < kClAtoms; ++t)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      pi[t][k] = rin[24 * c + 6 * t + k];
+      vi[t][k] = rin[24 * c + 6 * t + 3 + k];
+    }
+  // lane m < 12 writes value m: slot 4 c + m / 3, component m % 3
+  const int64_t o = 24 * c + 6 * (lane / 3) + 3 + lane % 3;
+  double xv = 0.0, zv = 0.0, av = 0.0;
+  if (lane < 3 * kClAtoms) {
+    xv = rin[o];
+    if (rz) zv = rz[o];
+    if (acc) av = acc[12 * c + lane];
+  }
+  double v[3 * kClAtoms];
+#pragma unroll
+  for (int q = 0; q < 3 * kClAtoms; ++q) v[q] = 0.0;
+  const int64_t estart = cl_ptr[c];
+  const uint32_t len = (uint32_t)(cl_ptr[c + 1] - estart);  // < 4096 (inch_anm_clusters)
+  const __amdgpu_buffer_rsrc_t rl = buf_rsrc(entries + estart, 4u * len);
+  const __amdgpu_buffer_rsrc_t rr = buf_rsrc(rin, (uint32_t)(192 * ncl));  // 48 bytes per slot < 2^32
+  uint32_t wn = __builtin_amdgcn_raw_buffer_load_b32(rl, (int)(4u * lane), 0, 0);
+  for (uint32_t e0 = 0; e0 < len; e0 += 64) {
+    const uint32_t w = wn;
+    const uint32_t off = 48u * (w >> 4);
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(rr, (int)off, 0, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(rr, (int)(off + 16), 0, 0);
+    const auto d = __builtin_amdgcn_raw_buffer_load_b128(rr, (int)(off + 32), 0, 0);
+    wn = __builtin_amdgcn_raw_buffer_load_b32(rl, (int)(4u * (e0 + 64 + lane)), 0, 0);
+    // the gathers and the next entries are in flight before any use
+    __builtin_amdgcn_sched_barrier(0);
+    const double pj0 = __hiloint2double((int)a[1], (int)a[0]), pj1 = __hiloint2double((int)a[3], (int)a[2]);
+    const double pj2 = __hiloint2double((int)b[1], (int)b[0]), xj0 = __hiloint2double((int)b[3], (int)b[2]);
+    const double xj1 = __hiloint2double((int)d[1], (int)d[0]), xj2 = __hiloint2double((int)d[3], (int)d[2]);
+#pragma unroll
+    for (int t = 0; t < kClAtoms; ++t) cl_term((w >> t) & 1u, pi[t], vi[t], pj0, pj1, pj2, xj0, xj1, xj2, v + 3 * t);
+  }
+  const double sum = cl_reduce(v, lane, red);
+  if (lane >= 3 * kClAtoms) return;
+  double out = sum + plus_i * xv;
+  if (c0 != 0.0) out -= c0 * xv;
+  out *= s1;
+  if (rz) out -= c2 * zv;
+  rout[o] = out;
+  if (acc) acc[12 * c + lane] = av + mu * out;
+}
+
+// Records from a vector in the matrix's order: rec's vector part of slot q = v of atom
+// cluster_atoms[q] (0 for a padding slot); acc (if given) = mu0 times the same, 3 per slot.
+__global__ void k_rec_pack(int64_t slots, const int32_t *__restrict__ cluster_atoms, const double *__restrict__ v,
+                           double *__restrict__ rec, double *__restrict__ acc, double mu0) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3 * slots) return;
+  const int64_t q = t / 3;
+  const int k = (int)(t % 3);
+  const int32_t at = cluster_atoms[q];
+  const double x = at >= 0 ? v[3 * (int64_t)at + k] : 0.0;
+  rec[6 * q + 3 + k] = x;
+  if (acc) acc[t] = mu0 * x;
+}
+
+// u of atom cluster_atoms[q] = acc of slot q (padding slots skipped).
+__global__ void k_rec_unpack(int64_t slots, const int32_t *__restrict__ cluster_atoms, const double *__restrict__ acc,
+                             double *__restrict__ u) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= 3 * slots) return;
+  const int32_t at = cluster_atoms[t / 3];
+  if (at >= 0) u[3 * (int64_t)at + t % 3] = acc[t];
 }
 
 // Several right-hand sides on the cluster layout: Y = (H + plus_i I) X, X and Y row-major
@@ -1397,6 +1488,41 @@ inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const
     INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: bad dtype");
   if (!ok) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_matfree_cl: config %d is not an unroll in {1,2,4} or 100 + clusters "
                                        "per wave in {4,8,16}, plus 2000 for XCD-contiguous clusters", code);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_anm_matfree_rec(int64_t ncl, const double *rec_in, const double *rec_z, double *rec_out,
+                                 const int64_t *cl_ptr, const uint32_t *entries, double plus_i, double s1, double c0,
+                                 double c2, double *acc, double mu, int32_t xcd, void *stream) {
+  INCH_REQUIRE(ncl > 0 && rec_in && rec_out && cl_ptr && entries && rec_in != rec_out && rec_z != rec_out,
+               "inch_anm_matfree_rec: bad args");
+  INCH_REQUIRE(192 * ncl <= (int64_t)UINT32_MAX, "inch_anm_matfree_rec: %lld clusters exceed the 32-bit offsets",
+               (long long)ncl);
+  hipStream_t st = (hipStream_t)stream;
+  if (xcd)
+    k_matfree_rec<true><<<(unsigned)ncl, 64, 0, st>>>(ncl, rec_in, rec_z, rec_out, cl_ptr, entries, plus_i, s1, c0,
+                                                      c2, acc, mu);
+  else
+    k_matfree_rec<false><<<(unsigned)ncl, 64, 0, st>>>(ncl, rec_in, rec_z, rec_out, cl_ptr, entries, plus_i, s1, c0,
+                                                       c2, acc, mu);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_rec_pack(int64_t ncl, const int32_t *cluster_atoms, const double *v, double *rec, double *acc,
+                          double mu0, void *stream) {
+  INCH_REQUIRE(ncl > 0 && cluster_atoms && v && rec, "inch_rec_pack: bad args");
+  const int64_t m = 3 * 4 * ncl;
+  k_rec_pack<<<(unsigned)((m + 255) / 256), 256, 0, (hipStream_t)stream>>>(4 * ncl, cluster_atoms, v, rec, acc, mu0);
+  HIP_TRY(hipGetLastError());
+  return INCH_OK;
+}
+
+inch_status inch_rec_unpack(int64_t ncl, const int32_t *cluster_atoms, const double *acc, double *u, void *stream) {
+  INCH_REQUIRE(ncl > 0 && cluster_atoms && acc && u, "inch_rec_unpack: bad args");
+  const int64_t m = 3 * 4 * ncl;
+  k_rec_unpack<<<(unsigned)((m + 255) / 256), 256, 0, (hipStream_t)stream>>>(4 * ncl, cluster_atoms, acc, u);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }

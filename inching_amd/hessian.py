@@ -93,6 +93,20 @@ class BlockHessian:
                     self._clusters = None
         return self._clusters
 
+    def records(self):
+        """Record form of the cluster layout for the Chebyshev recurrence (ops.anm_records:
+        three record buffers and the lists in slot numbers; built once), or None when the
+        cluster layout is unavailable or too large for 32-bit record offsets."""
+        if "_records" not in self.__dict__:
+            self._records = None
+            cl = self.clusters()
+            if cl is not None:
+                try:
+                    self._records = ops.anm_records(self.xyz, cl)
+                except ops.InchingError:
+                    self._records = None
+        return self._records
+
     def ensure(self, precision):
         """Make sure the values of ``precision`` exist (assembled on demand)."""
         if (precision == "f64" and self.vals64 is None) or (precision == "f32" and self.vals32 is None):

@@ -13,14 +13,25 @@ filtered matvec of degree m is m launches of the fused SpMV + three-term-recurre
 
 ``timer``: optional list; when set, every operator launch is bracketed by HIP events on
 the current stream and the event pairs are appended (kernel-time accounting).
+
+With the matrix-free float64 operator on the cluster layout, a filtered application runs
+its recurrence on records (``ops.anm_records``): the input is packed once into cluster order
+next to the coordinates, the degree products read and write records, and the accumulated
+result is unpacked once (``records=False`` or ``INCHING_RECORDS=0`` keeps the vectors in the
+matrix's order).
 """
+import os
+
 import torch
 
 from . import ops
 
+RECORDS = os.environ.get("INCHING_RECORDS", "1") != "0"
+
 
 class Operator:
-    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None, timer=None, tune=True):
+    def __init__(self, A, precision="f64", pol=None, Q=None, shift=0.0, lanes=None, timer=None, tune=True,
+                 records=None):
         if precision not in ("f32", "f64", "mf64"):
             raise ValueError("precision must be 'f32', 'f64' or 'mf64'")
         self.A = A
@@ -58,6 +69,13 @@ class Operator:
             self._acc = torch.empty(self.n, dtype=vdt, device=dev)
             self.mu = [float(x) for x in pol.AdjustedCoefficients[:pol.AdjustedDegree + 1]]
             self.degree = int(pol.AdjustedDegree)
+        self.rec = None
+        use_rec = RECORDS if records is None else bool(records)
+        if (pol is not None and precision == "mf64" and use_rec and getattr(A, "images", None) is None
+                and (lanes is None or int(lanes) // 1000 & 4)):
+            self.rec = A.records()
+            if self.rec is not None:
+                self._acc_rec = torch.empty(3 * A.clusters()[0].numel(), dtype=torch.float64, device=dev)
         if Q is not None:
             if Q.dtype != torch.float64 or Q.shape[1] != self.n:
                 raise ValueError("Q must be float64 [m, 3N]")
@@ -82,12 +100,49 @@ class Operator:
             self.timer.append((self.precision, epi, e0, e1, 1))
         self.n_spmv += 1
 
+    def _filtered_records(self, v64, u64):
+        """u = sum_k mu_k T_k((A - cc)/dd) v with the recurrence on records: v packed into
+        cluster order, three record buffers in turn (T_k, T_{k-1} read, T_{k+1} written),
+        the accumulator in slot order unpacked into u at the end."""
+        pol = self.pol
+        dd, cc = float(pol.dd), float(pol.cc)
+        recs, rent = self.rec
+        atoms, ptr = self.A.clusters()[:2]
+        acc = self._acc_rec
+        plus_i = self.A.plus_i
+        cur, prev = recs[0], None
+        free = [recs[1], recs[2]]
+        ops.rec_pack(atoms, v64, cur, acc, self.mu[0])
+        if self.timer is not None:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+        for k in range(1, self.degree + 1):
+            out = free.pop(0)
+            if k == 1:
+                ops.matfree_rec(cur, None, out, ptr, rent, plus_i, s1=1.0 / dd, c0=cc, acc=acc, mu=self.mu[1])
+            else:
+                ops.matfree_rec(cur, prev, out, ptr, rent, plus_i, s1=2.0 / dd, c0=cc, c2=1.0, acc=acc,
+                                mu=self.mu[k])
+            if prev is not None:
+                free.append(prev)
+            prev, cur = cur, out
+        if self.timer is not None:
+            e1.record()
+            self.timer.append(("mf64rec", "cheb", e0, e1, self.degree))
+        ops.rec_unpack(atoms, acc, u64)
+        self.n_spmv += self.degree
+
     def filtered(self, v64, u64):
         """u = sum_k mu_k T_k((A - cc)/dd) v.
 
         With a float64 operator the recurrence starts from v itself (it is only read) and,
         when u does not share v's memory, accumulates straight into u; otherwise v is
         converted into a scratch vector and the accumulator is converted into u at the end."""
+        if (self.rec is not None and v64.dtype == torch.float64 and u64.dtype == torch.float64
+                and v64.is_contiguous() and u64.is_contiguous()):
+            self._filtered_records(v64, u64)
+            return
         pol = self.pol
         dd, cc = float(pol.dd), float(pol.cc)
         scratch = [self._vk, self._vkm1, self._vkp1]

@@ -256,6 +256,47 @@ def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None):
     return atoms, ptr, base, entries[:nnz.value]
 
 
+def anm_records(xyz, clusters, copies=3, stream=None):
+    """Record form of the cluster layout (inch_anm_records): (list of ``copies`` record
+    buffers float64 [24 ncl], each with the slots' coordinates and a zero vector part,
+    rec_entries int32 [nnz] holding the uint32 words (slot << 4) | mask)."""
+    require_gpu()
+    _check_dev(xyz)
+    n = xyz.shape[0]
+    atoms, ptr, base, entries = clusters
+    ncl = (n + CLUSTER_ATOMS - 1) // CLUSTER_ATOMS
+    if atoms.numel() != CLUSTER_ATOMS * ncl or ptr.numel() != ncl + 1 or base.numel() != ncl:
+        raise ValueError("cluster layout does not match xyz")
+    rec = torch.empty(24 * ncl, dtype=torch.float64, device=xyz.device)
+    rent = torch.empty(max(1, entries.numel()), dtype=torch.int32, device=xyz.device)
+    call("inch_anm_records", n, P(xyz), ncl, P(atoms), P(ptr), P(base), P(entries), P(rec), P(rent),
+         stream_ptr(stream))
+    return [rec] + [rec.clone() for _ in range(copies - 1)], rent[:entries.numel()]
+
+
+def matfree_rec(rec_in, rec_z, rec_out, cl_ptr, rec_entries, plus_i, s1=1.0, c0=0.0, c2=0.0, acc=None, mu=0.0,
+                xcd=True, stream=None):
+    """One Chebyshev step on records (inch_anm_matfree_rec)."""
+    ncl = cl_ptr.numel() - 1
+    for r in (rec_in, rec_z, rec_out):
+        if r is not None and (r.numel() != 24 * ncl or r.dtype != torch.float64):
+            raise ValueError("records must be float64 [24 ncl]")
+    if acc is not None and (acc.numel() != 12 * ncl or acc.dtype != torch.float64):
+        raise ValueError("acc must be float64 [12 ncl]")
+    call("inch_anm_matfree_rec", ncl, P(rec_in), P(rec_z), P(rec_out), P(cl_ptr), P(rec_entries), float(plus_i),
+         float(s1), float(c0), float(c2), P(acc), float(mu), 1 if xcd else 0, stream_ptr(stream))
+
+
+def rec_pack(cluster_atoms, v, rec, acc=None, mu0=0.0, stream=None):
+    ncl = cluster_atoms.numel() // CLUSTER_ATOMS
+    call("inch_rec_pack", ncl, P(cluster_atoms), P(v), P(rec), P(acc), float(mu0), stream_ptr(stream))
+
+
+def rec_unpack(cluster_atoms, acc, u, stream=None):
+    ncl = cluster_atoms.numel() // CLUSTER_ATOMS
+    call("inch_rec_unpack", ncl, P(cluster_atoms), P(acc), P(u), stream_ptr(stream))
+
+
 def matfree_cl_apply(xyz, clusters, plus_i, X, Y, stream=None):
     """Y = (H + plus_i I) X on the cluster layout, float64 X [3n] or [3n, r] row-major."""
     n = xyz.shape[0]

@@ -215,7 +215,8 @@ inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *st
  * their smallest neighbour index (ties by spatial order), so consecutive clusters gather
  * from one band of the vectors; cluster_atoms int32[ncl * 4] (-1 pads the last group). Each cluster lists the union of its
  * atoms' rows once: entries uint32 = ((j - base[c]) << 8) | mask, ascending j, mask bit t set
- * when j is a neighbour of atom t of the cluster (j itself excluded); cl_ptr int64[ncl + 1],
+ * when j is a neighbour of atom t of the cluster at a nonzero distance (j itself and
+ * coincident atoms, whose terms the product masks, excluded); cl_ptr int64[ncl + 1],
  * base int32[ncl]. Two passes, as inch_neighbors_celllist: with entries == NULL it writes
  * cluster_atoms, cl_ptr, base and *nnz_out; the call with entries (capacity >= nnz) and the
  * arrays of the first pass writes the entries. INCH_ERR_CAPACITY when a cluster's rows hold
@@ -239,6 +240,30 @@ inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const
 inch_status inch_anm_matfree_cl_apply(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
                                       const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries,
                                       double plus_i, const double *x, double *y, int32_t nrhs, void *stream);
+/* Record form of the cluster layout for the Chebyshev filter's recurrence (replaces the
+ * recurrence vectors of ChebAv, Burn/PolynomialFilters/T2.py:28-92, around the product of
+ * Burn/Krylov/T3.py:54-239). Slot q = 4 c + t is atom t of cluster c. inch_anm_records
+ * writes rec float64 [24 ncl] (slot q: coordinates at 6q..6q+2, zeros for a padding slot;
+ * vector part 6q+3..6q+5 zeroed) and rec_entries uint32 [nnz of the layout] = (slot of j)
+ * << 4 | mask per cluster, ascending slot (cl_ptr unchanged). INCH_ERR_CAPACITY when
+ * 192 ncl exceeds 2^32 - 1 (32-bit record offsets). Syncs. */
+inch_status inch_anm_records(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
+                             const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double *rec,
+                             uint32_t *rec_entries, void *stream);
+/* One step of the recurrence on records: vector part of rec_out = s1 ((H + plus_i I) x - c0 x)
+ * - c2 z, with x, z the vector parts of rec_in, rec_z (rec_z may be NULL: no z term); acc
+ * float64 [12 ncl] (slot order, may be NULL) += mu times the result. rec_out's coordinate
+ * part is not written; rec_in's coordinates are read. xcd != 0 maps each XCD's workgroups to
+ * one contiguous range of clusters. */
+inch_status inch_anm_matfree_rec(int64_t ncl, const double *rec_in, const double *rec_z, double *rec_out,
+                                 const int64_t *cl_ptr, const uint32_t *rec_entries, double plus_i, double s1,
+                                 double c0, double c2, double *acc, double mu, int32_t xcd, void *stream);
+/* Vector part of rec for slot q = v[3 a .. 3 a + 2] with a = cluster_atoms[q] (0 for padding);
+ * acc (may be NULL) [12 ncl] = mu0 times the same. */
+inch_status inch_rec_pack(int64_t ncl, const int32_t *cluster_atoms, const double *v, double *rec, double *acc,
+                          double mu0, void *stream);
+/* u[3 a .. 3 a + 2] = acc of slot q for every non-padding slot q (a = cluster_atoms[q]). */
+inch_status inch_rec_unpack(int64_t ncl, const int32_t *cluster_atoms, const double *acc, double *u, void *stream);
 
 /* h[j] = sum_i V[j*ld + i] * u[i] for j < J (float64, deterministic two-phase
  * reduction). work: float64 scratch of inch_multidot_work_size(J) elements. */

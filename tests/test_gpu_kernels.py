@@ -571,6 +571,91 @@ def test_persistent_cluster_product_bitwise(mat, case):
                     assert torch.equal(acc, ref_acc), (cfg, dt, k)
 
 
+@pytest.mark.parametrize("case", ["globule2k", "padding_isolated"])
+def test_record_layout_and_product(mat, case):
+    """Record form of the cluster layout: the records hold each slot's coordinates, the lists
+    name the same (neighbour, mask) pairs as the cluster lists in slot numbers, ascending;
+    one recurrence step on records equals the cluster kernel's product with the same
+    epilogue (float64, 1e-13 relative: only the order of the sum over neighbours differs),
+    also with the padding slot and the empty lists of isolated atoms."""
+    from inching_amd import ops
+    from inching_amd.hessian import build_hessian
+    if case == "globule2k":
+        A, _ = mat
+    else:
+        X = small_globule(1003, seed=23)
+        far = np.stack([np.full(8, 500.0), np.full(8, 500.0), 60.0 * np.arange(8)], axis=1)
+        A = build_hessian(np.concatenate([X, far]), 8.0, plus_i=1.0, reorder=True, precision=())
+    _print("case: record layout and product, %s, n=%d" % (case, A.n_atoms))
+    cl = A.clusters()
+    recs, rent = A.records()
+    atoms = cl[0].cpu().numpy()
+    ncl = atoms.size // 4
+    slot_of = np.full(A.n_atoms, -1, np.int64)
+    slot_of[atoms[atoms >= 0]] = np.nonzero(atoms >= 0)[0]
+    rec = recs[0].cpu().numpy().reshape(-1, 6)
+    xyz = A.xyz.cpu().numpy()
+    assert np.array_equal(rec[atoms >= 0, :3], xyz[atoms[atoms >= 0]]) and not rec[atoms < 0].any()
+    assert not rec[:, 3:].any()
+    _, lists = _decode_clusters(cl)
+    ptr = cl[1].cpu().numpy()
+    w = rent.cpu().numpy().view(np.uint32).astype(np.int64)
+    for c, (j, m) in enumerate(lists):
+        e = w[ptr[c]:ptr[c + 1]]
+        assert np.all(np.diff(e >> 4) > 0)
+        got = sorted(zip((e >> 4).tolist(), (e & 15).tolist()))
+        assert got == sorted(zip(slot_of[j].tolist(), m.tolist())), c
+    # one step with every epilogue term, against the cluster kernel in the matrix's order
+    nd = A.shape[0]
+    rng = np.random.default_rng(12)
+    x, z, a0 = (rng.standard_normal(nd) for _ in range(3))
+    y_ref = torch.empty(nd, dtype=torch.float64, device="cuda")
+    acc_ref = _dev(a0)
+    A.matfree_spmv(_dev(x), y_ref, config=6001, s1=0.3, c0=2.0, z=_dev(z), c2=1.0, acc=acc_ref, mu=0.5)
+    rin, rz, rout = recs
+    acc = torch.empty(12 * ncl, dtype=torch.float64, device="cuda")
+    ops.rec_pack(cl[0], _dev(a0), rout, acc, 1.0)          # acc = a0 in slot order
+    ops.rec_pack(cl[0], _dev(z), rz)
+    ops.rec_pack(cl[0], _dev(x), rin)
+    for xcd in (True, False):
+        acc_s = acc.clone()
+        ops.matfree_rec(rin, rz, rout, cl[1], rent, A.plus_i, s1=0.3, c0=2.0, c2=1.0, acc=acc_s, mu=0.5, xcd=xcd)
+        y = torch.full((nd,), float("nan"), dtype=torch.float64, device="cuda")
+        yv = rout.view(-1, 6)[:, 3:].contiguous().view(-1)
+        ops.rec_unpack(cl[0], yv, y)
+        scale = float(y_ref.abs().max())
+        assert float((y - y_ref).abs().max()) <= 1e-13 * scale, xcd
+        acc_back = torch.full((nd,), float("nan"), dtype=torch.float64, device="cuda")
+        ops.rec_unpack(cl[0], acc_s, acc_back)
+        assert float((acc_back - acc_ref).abs().max()) <= 1e-13 * max(1.0, scale), xcd
+        # padding slots stay zero
+        pad = torch.as_tensor(np.repeat(atoms < 0, 3), device="cuda")
+        assert not bool(yv[pad].any()) and not bool(acc_s[pad].any() if pad.any() else False)
+
+
+def test_record_filter_matches_matrix_order_filter(mat):
+    """A filtered application p(A) v with the recurrence on records equals the same filter
+    with the vectors in the matrix's order (float64, 1e-12 relative), also in place (u = v)."""
+    from inching_amd import chebyshev as CH
+    from inching_amd.operator import Operator
+    A, _ = mat
+    _print("case: record filter vs matrix-order filter")
+    lo, hi = CH.spectrum_bound(A, precision="mf64", stop_rel=CH.BOUND_STOP_REL)
+    pol = CH.design_filter(A.plus_i, A.plus_i + 0.1 * (hi - A.plus_i), (A.plus_i, hi * 1.001), 5120, 5, 0.7)
+    op_r = Operator(A, "mf64", pol=pol, records=True)
+    op_m = Operator(A, "mf64", pol=pol, records=False)
+    assert op_r.rec is not None and op_m.rec is None
+    v = torch.randn(A.shape[0], dtype=torch.float64, device="cuda")
+    ur, um = torch.empty_like(v), torch.empty_like(v)
+    op_r.apply(v, ur)
+    op_m.apply(v, um)
+    s = float(um.abs().max())
+    assert float((ur - um).abs().max()) <= 1e-12 * s, (float((ur - um).abs().max()), s, pol.AdjustedDegree)
+    w = v.clone()
+    op_r.apply(w, w)
+    assert float((w - um).abs().max()) <= 1e-12 * s
+
+
 def test_cluster_capacity_falls_back():
     """A cluster whose rows hold more than 4096 entries makes the layout unavailable (a host
     error, no kernel fault); the operator then keeps the per-row kernels."""
