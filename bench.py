@@ -100,6 +100,15 @@ def spmv_only(args):
         "matfree_f64_cheb": lambda: A.matfree_spmv(x64, y64, s1=0.1, c0=2.0, z=z64, c2=1.0, acc=acc64, mu=0.01,
                                                    config=cfg["mf64"]),
     }
+    if A.records() is not None:
+        (r0, r1, r2), rent = A.records()
+        cl = A.clusters()
+        acc_r = torch.zeros(3 * cl[0].numel(), dtype=torch.float64, device="cuda")
+        ops.rec_pack(cl[0], x64, r0)
+        ops.rec_pack(cl[0], z64, r1)
+        cases["matfree_rec_f64_cheb"] = lambda: ops.matfree_rec(r0, r1, r2, cl[1], rent, A.plus_i, s1=0.1, c0=2.0,
+                                                                c2=1.0, acc=acc_r, mu=0.01)
+        out["bytes_per_launch"]["k_matfree_rec<true>"] = int(kernel_bytes("mf64rec", A, "cheb"))
     for name, fn in cases.items():
         for _ in range(3):
             fn()

@@ -345,8 +345,10 @@ class KPMCounter:
     Unlike the reference's estimate (which damps with the filter's kernel, ~1 for low
     degrees, so narrow intervals at the end of a large spectrum give oscillating, even
     negative, counts), this one is monotone in r and non-negative; its resolution near the
-    lower end of the spectrum is about dd (pi / degree)^2 / 2 in eigenvalue. Each moment is
-    one operator launch with the fused Chebyshev step plus one dot."""
+    lower end of the spectrum is about dd (pi / degree)^2 / 2 in eigenvalue. The moments
+    come from degree / 2 operator launches (fused Chebyshev steps) by the doubling relations
+    mu_2k = 2 <T_k v, T_k v> - mu_0 and mu_2k-1 = 2 <T_k v, T_k-1 v> - mu_1 (two dots per
+    step; an odd degree takes one more launch)."""
 
     def __init__(self, A, bound, degree=64, probes=1, seed=0, precision="mf64"):
         from .autotune import best_config, bsr_launch
@@ -370,18 +372,19 @@ class KPMCounter:
 
         g = torch.Generator(device=dev)
         g.manual_seed(int(seed))
-        mom = torch.zeros((probes, self.degree + 1), dtype=torch.float64, device=dev)
         ws = ops.Workspace(1, dev)
         t = [torch.empty(self.n, dtype=vdt, device=dev) for _ in range(3)]
         d64 = torch.empty(self.n, dtype=torch.float64, device=dev) if vdt != torch.float64 else None
+        half = (self.degree + 1) // 2
+        d64p = torch.empty(self.n, dtype=torch.float64, device=dev) if vdt != torch.float64 else None
+        dots = torch.zeros((probes, 2 * half + 1), dtype=torch.float64, device=dev)
         for p in range(probes):
             v = torch.randn(self.n, generator=g, dtype=torch.float64, device=dev)
             v /= torch.linalg.norm(v)
-            vr = v.view(1, -1)
-            mom[p, 0] = 1.0
             t[0].copy_(v)
             prev, cur, nxt = None, t[0], t[1]
-            for k in range(1, self.degree + 1):
+            prev64 = v                          # T_{k-1} v in float64
+            for k in range(1, half + 1):
                 if k == 1:
                     step(cur, nxt, s1=1.0 / self.dd, c0=self.cc)
                 else:
@@ -390,12 +393,28 @@ class KPMCounter:
                 if d64 is not None:
                     ops.convert(nxt, d64)
                     src = d64
-                ops.multidot(vr, 1, src, mom[p, k:k + 1], ws)
+                # dots[2k] = <T_k, T_k>, dots[2k - 1] = <T_k, T_{k-1}>
+                ops.multidot(src.view(1, -1), 1, src, dots[p, 2 * k:2 * k + 1], ws)
+                ops.multidot(prev64.view(1, -1), 1, src, dots[p, 2 * k - 1:2 * k], ws)
+                if d64 is not None:
+                    d64, d64p = d64p, d64
+                    prev64 = d64p
+                else:
+                    prev64 = src
                 free = prev if prev is not None else t[2]
                 prev, cur, nxt = cur, nxt, free
-        self.moments = mom.mean(0).cpu().numpy()
+        dh = dots.cpu().numpy()
+        mom = np.empty((probes, self.degree + 1))
+        mom[:, 0] = 1.0
+        mom[:, 1] = dh[:, 1]
+        for k in range(1, half + 1):
+            if 2 * k <= self.degree:
+                mom[:, 2 * k] = 2.0 * dh[:, 2 * k] - 1.0
+            if 2 <= 2 * k - 1 <= self.degree:
+                mom[:, 2 * k - 1] = 2.0 * dh[:, 2 * k - 1] - mom[:, 1]
+        self.moments = mom.mean(0)
         self.jackson = jackson_kernel(self.degree)
-        self.n_spmv = probes * self.degree
+        self.n_spmv = probes * half
 
     def count(self, r):
         x = min(1.0, max(-1.0, (float(r) - self.cc) / self.dd))

@@ -182,9 +182,15 @@ __device__ int32_t cc_find(int32_t *parent, int32_t v) {
   return par;
 }
 
-__global__ void k_cc_init(int64_t n, int32_t *__restrict__ parent) {
+// parent[v] = v's smallest neighbour when that is smaller than v (a valid hook: parent[v] <=
+// v and in v's component), which leaves most trees shallow before any CAS
+__global__ void k_cc_init(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+                          int32_t *__restrict__ parent) {
   const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (v < n) parent[v] = (int32_t)v;
+  if (v >= n) return;
+  int32_t m = (int32_t)v;
+  for (int64_t b = indptr[v]; b < indptr[v + 1]; ++b) m = indices[b] < m ? indices[b] : m;
+  parent[v] = m;
 }
 
 __global__ void k_cc_hook(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
@@ -259,13 +265,26 @@ __global__ void k_cc_seeds(int64_t n, const int64_t *__restrict__ seeds, const i
   }
 }
 
-__global__ void k_cc_vertex(int64_t n, const int32_t *__restrict__ label, const int32_t *__restrict__ croot,
-                            int32_t *__restrict__ cv, int32_t *__restrict__ csize) {
+// cv[v] = component of v; csize[c] += 1 per node, one atomic per run of a wave's lanes that
+// share a component (nodes of a component tend to be consecutive: few counters, heavy
+// contention otherwise)
+__global__ void __launch_bounds__(256) k_cc_vertex(int64_t n, const int32_t *__restrict__ label,
+                                                   const int32_t *__restrict__ croot, int32_t *__restrict__ cv,
+                                                   int32_t *__restrict__ csize) {
   const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  const int32_t c = croot[label[v]];
-  cv[v] = c;
-  atomicAdd(&csize[c], 1);
+  const int lane = threadIdx.x & 63;
+  const int32_t c = v < n ? croot[label[v]] : -1;
+  if (v < n) cv[v] = c;
+  const int32_t cp = __shfl_up(c, 1, 64);
+  const bool head = c >= 0 && (lane == 0 || cp != c);
+  // run length: lanes up to the next head (or the wave's end)
+  const uint64_t heads = __ballot(head);
+  if (head) {
+    const uint64_t later = lane < 63 ? heads >> (lane + 1) : 0;
+    int run = later ? __ffsll((unsigned long long)later) : 64 - lane;
+    if (v + run > n) run = (int)(n - v);
+    atomicAdd(&csize[c], run);
+  }
 }
 
 // round 0: the seeds, one per component, each first in its component
@@ -283,15 +302,17 @@ __global__ void k_rcm_seed_level(int32_t C, const int32_t *__restrict__ S, const
   lcnt[c] = 0;
 }
 
-// final positions of a new level [e, e + L1): its nodes of component c form one segment,
-// whose first node follows the done[c] nodes of c's earlier levels
+// final positions of a new level [e, e + L1): its nodes of component c form one segment
+// (the frontier is grouped by component, in component order, so their children are too),
+// whose first node follows the done[c] nodes of c's earlier levels. Segment bounds come from
+// comparing neighbouring positions: one writer each, no atomics.
 __global__ void k_rcm_seg_first(int32_t e, int32_t L1, const int32_t *__restrict__ ord, const int32_t *__restrict__ cv,
                                 int32_t *__restrict__ first, int32_t *__restrict__ lcnt) {
   const int32_t q = e + blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= e + L1) return;
   const int32_t c = cv[ord[q]];
-  atomicMin(&first[c], q);
-  atomicAdd(&lcnt[c], 1);
+  if (q == e || cv[ord[q - 1]] != c) first[c] = q;
+  if (q == e + L1 - 1 || cv[ord[q + 1]] != c) lcnt[c] = q + 1;   // one past the segment's end
 }
 
 __global__ void k_rcm_seg_fin(int32_t e, int32_t L1, const int32_t *__restrict__ ord, const int32_t *__restrict__ cv,
@@ -308,11 +329,7 @@ __global__ void k_rcm_seg_close(int32_t e, int32_t L1, const int32_t *__restrict
   const int32_t q = e + blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= e + L1) return;
   const int32_t c = cv[ord[q]];
-  if (first[c] == q) {
-    done[c] += lcnt[c];
-    first[c] = INT_MAX;
-    lcnt[c] = 0;
-  }
+  if (first[c] == q) done[c] += lcnt[c] - q;
 }
 
 __global__ void k_rcm_finish(int64_t n, const int32_t *__restrict__ ord, const int32_t *__restrict__ fin,
@@ -406,7 +423,7 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
   INCH_REQUIRE(!bad_h, "inch_rcm_device: seeds is not a permutation");
   k_rcm_init<<<blocks_for(n, bs), bs, 0, st>>>(n, pos, claim);
   // components, their seeds (first node in the seed order) and sizes, in seed order
-  k_cc_init<<<blocks_for(n, bs), bs, 0, st>>>(n, label);
+  k_cc_init<<<blocks_for(n, bs), bs, 0, st>>>(n, indptr, indices, label);
   k_cc_hook<<<blocks_for(n, bs), bs, 0, st>>>(n, indptr, indices, label);
   k_cc_flatten<<<blocks_for(n, bs), bs, 0, st>>>(n, label);
   k_cc_rank<<<blocks_for(n, bs), bs, 0, st>>>(n, seeds, rank, minr);

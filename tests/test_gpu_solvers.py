@@ -173,6 +173,32 @@ def test_device_kpm_bisection_interval_holds_wanted_count(problem, designated):
     assert abs(exact - wanted) <= margin + 8, (exact, interval, cnt)
 
 
+def test_kpm_moments_by_doubling_match_the_recurrence(problem):
+    """The counter's moments (degree / 2 launches and the doubling relations) equal v^T T_k(A') v
+    from the plain three-term recurrence on the dense matrix for the same probe (1e-10), for
+    an even and an odd degree."""
+    import torch
+    from inching_amd import chebyshev as CH
+    A, Ad, w, U, Q = problem
+    print("case: KPM moments by doubling", flush=True)
+    bound = (1.0, w[-1] * (1 + 1e-3))
+    for degree in (64, 33):
+        c = CH.KPMCounter(A, bound, degree=degree, probes=1, seed=11, precision="mf64")
+        g = torch.Generator(device=A.device)
+        g.manual_seed(11)
+        v = torch.randn(A.shape[0], generator=g, dtype=torch.float64, device=A.device)
+        v = (v / torch.linalg.norm(v)).cpu().numpy()
+        cc, dd = 0.5 * (bound[1] + bound[0]), 0.5 * (bound[1] - bound[0])
+        Ap = (Ad - cc * np.eye(Ad.shape[0])) / dd
+        t0, t1 = v, Ap @ v
+        ref = [v @ t0, v @ t1]
+        for _ in range(2, degree + 1):
+            t0, t1 = t1, 2.0 * (Ap @ t1) - t0
+            ref.append(v @ t1)
+        assert c.n_spmv == (degree + 1) // 2
+        assert np.max(np.abs(np.asarray(ref) - c.moments)) <= 1e-10, degree
+
+
 @pytest.mark.parametrize("precision", ["mf64", "f32"])
 def test_kpm_counter_matches_exact_counts(problem, precision):
     """The Jackson-damped KPM count of eigenvalues in [PlusI, r] against the exact count from
