@@ -455,8 +455,10 @@ __device__ __forceinline__ double cl_pair_r2(const double *__restrict__ xyz, int
   return __builtin_fma(d2, d2, __builtin_fma(d1, d1, d0 * d0));
 }
 
-// One block per cluster. The rows of its atoms are gathered as keys (j << 3 | t), sorted
-// (bitonic, LDS), and each run of equal j becomes one entry with the OR of its atom bits.
+// One block per cluster. The rows of its atoms (each in ascending column order, as the
+// neighbour search and the renumbering write them) are merged into keys (j << 3 | t) in
+// ascending order in LDS, and each run of equal j becomes one entry with the OR of its atom
+// bits.
 // Count pass (FILL = false): counts[c] = entries, base[c] = smallest j; overflow bit 1 =
 // more than kMaxUnion row entries, bit 2 = the union spans 2^24 columns or more.
 // Fill pass: entries[cl_ptr[c] + r] = ((j - base[c]) << 8) | mask, ascending j.
@@ -500,35 +502,38 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const double *__r
     return;
   }
   const int total = (int)total64;
-  int p2 = 1;
-  while (p2 < total) p2 <<= 1;
-  for (int e = tid; e < p2; e += blockDim.x) {
-    uint64_t k = ~0ull;
-    if (e < total) {
-      int t = 0;
-      while (e >= rl[t + 1]) ++t;
-      const int32_t j = col[rb[t] + (e - rl[t])];
-      k = ((uint64_t)(uint32_t)j << 3) | (uint64_t)t;
-    }
-    key[e] = k;
+  // the four rows (ascending columns) into val[]; each entry's place in the merged order is
+  // its index in its own row plus, for every other row, the entries below it (equal columns
+  // of earlier rows count as below), found by binary search: the keys (j << 3 | t) come out
+  // in ascending order, as a sort of them would give. A row out of order sets overflow bit 4.
+  for (int e = tid; e < total; e += blockDim.x) {
+    int t = 0;
+    while (e >= rl[t + 1]) ++t;
+    val[e] = col[rb[t] + (e - rl[t])];
   }
   __syncthreads();
-  for (int k = 2; k <= p2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int t = tid; t < p2; t += blockDim.x) {
-        const int ixj = t ^ j;
-        if (ixj > t) {
-          const bool up = (t & k) == 0;
-          const uint64_t a = key[t], b = key[ixj];
-          if ((a > b) == up) {
-            key[t] = b;
-            key[ixj] = a;
-          }
-        }
+  for (int e = tid; e < total; e += blockDim.x) {
+    int t = 0;
+    while (e >= rl[t + 1]) ++t;
+    const int32_t j = val[e];
+    if (e > rl[t] && val[e - 1] >= j) atomicOr(overflow, 4);
+    int rank = e - (int)rl[t];
+    for (int u = 0; u < kClusterAtoms; ++u) {
+      if (u == t) continue;
+      int lo = (int)rl[u], hi = (int)rl[u + 1];
+      while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        const int32_t x = val[m];
+        if (u < t ? x <= j : x < j)
+          lo = m + 1;
+        else
+          hi = m;
       }
-      __syncthreads();
+      rank += lo - (int)rl[u];
     }
+    key[rank] = ((uint64_t)(uint32_t)j << 3) | (uint64_t)t;
   }
+  __syncthreads();
   // runs of equal j: the run's first element carries the mask (0 = nothing to emit)
   const int chunk = (total + 255) / 256;
   const int e0 = tid * chunk, e1 = min(total, e0 + chunk);
@@ -936,6 +941,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
     HIP_TRY(hipStreamSynchronize(st));
     if (over & 1) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters (%s): a cluster's rows hold more than %d entries", pass, kMaxUnion);
     if (over & 2) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters (%s): a cluster's neighbours span 2^24 columns or more", pass);
+    if (over & 4) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_clusters (%s): a row's columns are not in ascending order", pass);
     return INCH_OK;
   };
   if (entries) {  // fill pass on the layout of the count pass
