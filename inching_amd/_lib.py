@@ -48,7 +48,8 @@ SIGNATURES = {
     "inch_anm_clusters": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "inch_anm_matfree_cl_apply": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _i32, _vp]),
     "inch_anm_records": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "inch_anm_matfree_rec": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _vp, _dbl, _i32, _vp]),
+    "inch_anm_matfree_rec": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _vp, _dbl, _i32,
+                                       _vp]),
     "inch_rec_pack": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _dbl, _vp]),
     "inch_rec_unpack": (_STATUS, [_i64, _vp, _vp, _vp, _vp]),
     "inch_anm_matfree_cl": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _i32, _vp, _vp, _dbl, _dbl, _vp,

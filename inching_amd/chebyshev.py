@@ -458,9 +458,18 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64", stop_rel=None, inf
         A.ensure(precision)
     n = A.shape[0]
     dev = A.device
+    # with the matrix-free operator the run can stay in the record layout's slot order (the
+    # Ritz values do not depend on the numbering): vectors of 12 ncl entries, zero on
+    # padding slots (whose rows are zero), one pack into a record per product
+    rec = A.records() if precision == "mf64" and getattr(A, "images", None) is None else None
+    if rec is not None:
+        atoms, ptr = A.clusters()[:2]
+        n = 3 * atoms.numel()
     g = torch.Generator(device=dev)
     g.manual_seed(int(seed))
     v = torch.rand(n, generator=g, dtype=torch.float64, device=dev) - 0.5
+    if rec is not None:
+        v.view(-1, 3)[atoms < 0] = 0.0
     v /= torch.linalg.norm(v)
     f = torch.empty_like(v)
     v0 = torch.empty_like(v)
@@ -471,11 +480,14 @@ def spectrum_bound(A, n_tridiag=300, seed=0, precision="f64", stop_rel=None, inf
     row = v.view(1, -1)
 
     from .autotune import best_config, bsr_launch
-    cfg = best_config(A, precision)
+    cfg = best_config(A, precision) if rec is None else 0
     blanes, c16 = bsr_launch(A, cfg) if precision != "mf64" else (0, None)
 
     def av(x, y):
-        if vals is None:
+        if rec is not None:
+            ops.rec_pack(None, x, rec[0][0], ncl=ptr.numel() - 1)
+            ops.matfree_rec(rec[0][0], None, None, ptr, rec[1], A.plus_i, y=y)
+        elif vals is None:
             A.matfree_spmv(x, y, config=cfg)
         else:
             ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=blanes, col16=c16)

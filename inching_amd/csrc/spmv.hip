@@ -583,13 +583,14 @@ k_matfree_clp(int64_t n, const double *__restrict__ xyz, int64_t ncl, const int3
 // adjacent records: on the 2.4M-atom assembly this takes the product from 0.79 to 0.57 ms
 // (gathers from the coordinate and vector arrays in the matrix's order touch about twice
 // the cache lines). The epilogue is that of k_matfree_cl: rout's vector part = s1 (A x +
-// (plus_i - c0) x) - c2 z with x and z the vector parts of rin and rz, acc += mu rout.
-// rout's coordinate part is not written (the caller's buffers carry the coordinates).
+// (plus_i - c0) x) - c2 z with x and z the vector parts of rin and rz, acc += mu rout;
+// the result also goes to yout (slot order, 12 per cluster) when given. rout's coordinate
+// part is not written (the caller's buffers carry the coordinates).
 template <bool XCD>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 k_matfree_rec(int64_t ncl, const double *__restrict__ rin, const double *__restrict__ rz, double *__restrict__ rout,
-              const int64_t *__restrict__ cl_ptr, const uint32_t *__restrict__ entries, double plus_i, double s1,
-              double c0, double c2, double *__restrict__ acc, double mu) {
+              double *__restrict__ yout, const int64_t *__restrict__ cl_ptr, const uint32_t *__restrict__ entries,
+              double plus_i, double s1, double c0, double c2, double *__restrict__ acc, double mu) {
   const int64_t c = xcd_block(XCD);
   if (c >= ncl) return;
   const int lane = threadIdx.x;
@@ -639,19 +640,21 @@ k_matfree_rec(int64_t ncl, const double *__restrict__ rin, const double *__restr
   if (c0 != 0.0) out -= c0 * xv;
   out *= s1;
   if (rz) out -= c2 * zv;
-  rout[o] = out;
+  if (rout) rout[o] = out;
+  if (yout) yout[12 * c + lane] = out;
   if (acc) acc[12 * c + lane] = av + mu * out;
 }
 
 // Records from a vector in the matrix's order: rec's vector part of slot q = v of atom
-// cluster_atoms[q] (0 for a padding slot); acc (if given) = mu0 times the same, 3 per slot.
+// cluster_atoms[q] (0 for a padding slot), or v of slot q itself when cluster_atoms is null
+// (a vector already in slot order); acc (if given) = mu0 times the same, 3 per slot.
 __global__ void k_rec_pack(int64_t slots, const int32_t *__restrict__ cluster_atoms, const double *__restrict__ v,
                            double *__restrict__ rec, double *__restrict__ acc, double mu0) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= 3 * slots) return;
   const int64_t q = t / 3;
   const int k = (int)(t % 3);
-  const int32_t at = cluster_atoms[q];
+  const int32_t at = cluster_atoms ? cluster_atoms[q] : (int32_t)q;
   const double x = at >= 0 ? v[3 * (int64_t)at + k] : 0.0;
   rec[6 * q + 3 + k] = x;
   if (acc) acc[t] = mu0 * x;
@@ -1492,27 +1495,28 @@ inch_status inch_anm_matfree_cl(int64_t n, const double *xyz, int64_t ncl, const
   return INCH_OK;
 }
 
-inch_status inch_anm_matfree_rec(int64_t ncl, const double *rec_in, const double *rec_z, double *rec_out,
+inch_status inch_anm_matfree_rec(int64_t ncl, const double *rec_in, const double *rec_z, double *rec_out, double *y,
                                  const int64_t *cl_ptr, const uint32_t *entries, double plus_i, double s1, double c0,
                                  double c2, double *acc, double mu, int32_t xcd, void *stream) {
-  INCH_REQUIRE(ncl > 0 && rec_in && rec_out && cl_ptr && entries && rec_in != rec_out && rec_z != rec_out,
+  INCH_REQUIRE(ncl > 0 && rec_in && (rec_out || y) && cl_ptr && entries && rec_in != rec_out &&
+                   (!rec_z || rec_z != rec_out),
                "inch_anm_matfree_rec: bad args");
   INCH_REQUIRE(192 * ncl <= (int64_t)UINT32_MAX, "inch_anm_matfree_rec: %lld clusters exceed the 32-bit offsets",
                (long long)ncl);
   hipStream_t st = (hipStream_t)stream;
   if (xcd)
-    k_matfree_rec<true><<<(unsigned)ncl, 64, 0, st>>>(ncl, rec_in, rec_z, rec_out, cl_ptr, entries, plus_i, s1, c0,
-                                                      c2, acc, mu);
+    k_matfree_rec<true><<<(unsigned)ncl, 64, 0, st>>>(ncl, rec_in, rec_z, rec_out, y, cl_ptr, entries, plus_i, s1,
+                                                      c0, c2, acc, mu);
   else
-    k_matfree_rec<false><<<(unsigned)ncl, 64, 0, st>>>(ncl, rec_in, rec_z, rec_out, cl_ptr, entries, plus_i, s1, c0,
-                                                       c2, acc, mu);
+    k_matfree_rec<false><<<(unsigned)ncl, 64, 0, st>>>(ncl, rec_in, rec_z, rec_out, y, cl_ptr, entries, plus_i, s1,
+                                                       c0, c2, acc, mu);
   HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
 
 inch_status inch_rec_pack(int64_t ncl, const int32_t *cluster_atoms, const double *v, double *rec, double *acc,
                           double mu0, void *stream) {
-  INCH_REQUIRE(ncl > 0 && cluster_atoms && v && rec, "inch_rec_pack: bad args");
+  INCH_REQUIRE(ncl > 0 && v && rec, "inch_rec_pack: bad args");
   const int64_t m = 3 * 4 * ncl;
   k_rec_pack<<<(unsigned)((m + 255) / 256), 256, 0, (hipStream_t)stream>>>(4 * ncl, cluster_atoms, v, rec, acc, mu0);
   HIP_TRY(hipGetLastError());

@@ -275,21 +275,25 @@ def anm_records(xyz, clusters, copies=3, stream=None):
 
 
 def matfree_rec(rec_in, rec_z, rec_out, cl_ptr, rec_entries, plus_i, s1=1.0, c0=0.0, c2=0.0, acc=None, mu=0.0,
-                xcd=True, stream=None):
-    """One Chebyshev step on records (inch_anm_matfree_rec)."""
+                xcd=True, y=None, stream=None):
+    """One Chebyshev step on records (inch_anm_matfree_rec); ``y``: optional slot-order output."""
     ncl = cl_ptr.numel() - 1
     for r in (rec_in, rec_z, rec_out):
         if r is not None and (r.numel() != 24 * ncl or r.dtype != torch.float64):
             raise ValueError("records must be float64 [24 ncl]")
-    if acc is not None and (acc.numel() != 12 * ncl or acc.dtype != torch.float64):
-        raise ValueError("acc must be float64 [12 ncl]")
-    call("inch_anm_matfree_rec", ncl, P(rec_in), P(rec_z), P(rec_out), P(cl_ptr), P(rec_entries), float(plus_i),
-         float(s1), float(c0), float(c2), P(acc), float(mu), 1 if xcd else 0, stream_ptr(stream))
+    for t in (acc, y):
+        if t is not None and (t.numel() != 12 * ncl or t.dtype != torch.float64):
+            raise ValueError("acc and y must be float64 [12 ncl]")
+    call("inch_anm_matfree_rec", ncl, P(rec_in), P(rec_z), P(rec_out), P(y), P(cl_ptr), P(rec_entries),
+         float(plus_i), float(s1), float(c0), float(c2), P(acc), float(mu), 1 if xcd else 0, stream_ptr(stream))
 
 
-def rec_pack(cluster_atoms, v, rec, acc=None, mu0=0.0, stream=None):
-    ncl = cluster_atoms.numel() // CLUSTER_ATOMS
-    call("inch_rec_pack", ncl, P(cluster_atoms), P(v), P(rec), P(acc), float(mu0), stream_ptr(stream))
+def rec_pack(cluster_atoms, v, rec, acc=None, mu0=0.0, stream=None, ncl=None):
+    """Vector part of the records from v (matrix order through cluster_atoms, or slot order
+    when cluster_atoms is None, then ``ncl`` is required)."""
+    if cluster_atoms is not None:
+        ncl = cluster_atoms.numel() // CLUSTER_ATOMS
+    call("inch_rec_pack", int(ncl), P(cluster_atoms), P(v), P(rec), P(acc), float(mu0), stream_ptr(stream))
 
 
 def rec_unpack(cluster_atoms, acc, u, stream=None):

@@ -250,16 +250,18 @@ inch_status inch_anm_matfree_cl_apply(int64_t n, const double *xyz, int64_t ncl,
 inch_status inch_anm_records(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
                              const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double *rec,
                              uint32_t *rec_entries, void *stream);
-/* One step of the recurrence on records: vector part of rec_out = s1 ((H + plus_i I) x - c0 x)
- * - c2 z, with x, z the vector parts of rec_in, rec_z (rec_z may be NULL: no z term); acc
- * float64 [12 ncl] (slot order, may be NULL) += mu times the result. rec_out's coordinate
+/* One step of the recurrence on records: result = s1 ((H + plus_i I) x - c0 x) - c2 z, with
+ * x, z the vector parts of rec_in, rec_z (rec_z may be NULL: no z term), written to the vector
+ * part of rec_out and/or to y (float64 [12 ncl], slot order; either may be NULL, not both);
+ * acc float64 [12 ncl] (slot order, may be NULL) += mu times the result. rec_out's coordinate
  * part is not written; rec_in's coordinates are read. xcd != 0 maps each XCD's workgroups to
  * one contiguous range of clusters. */
-inch_status inch_anm_matfree_rec(int64_t ncl, const double *rec_in, const double *rec_z, double *rec_out,
+inch_status inch_anm_matfree_rec(int64_t ncl, const double *rec_in, const double *rec_z, double *rec_out, double *y,
                                  const int64_t *cl_ptr, const uint32_t *rec_entries, double plus_i, double s1,
                                  double c0, double c2, double *acc, double mu, int32_t xcd, void *stream);
-/* Vector part of rec for slot q = v[3 a .. 3 a + 2] with a = cluster_atoms[q] (0 for padding);
- * acc (may be NULL) [12 ncl] = mu0 times the same. */
+/* Vector part of rec for slot q = v[3 a .. 3 a + 2] with a = cluster_atoms[q] (0 for padding),
+ * or v[3 q .. 3 q + 2] when cluster_atoms is NULL (v in slot order, [12 ncl]); acc (may be
+ * NULL) [12 ncl] = mu0 times the same. */
 inch_status inch_rec_pack(int64_t ncl, const int32_t *cluster_atoms, const double *v, double *rec, double *acc,
                           double mu0, void *stream);
 /* u[3 a .. 3 a + 2] = acc of slot q for every non-padding slot q (a = cluster_atoms[q]). */
