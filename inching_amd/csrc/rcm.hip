@@ -161,12 +161,15 @@ __global__ void k_rcm_place(int32_t e, int32_t L1, const int32_t *__restrict__ o
 // ---- components: union-find over the edges. A root is only ever hooked (CAS) onto a
 // smaller root, so parent[v] <= v throughout and each component ends with its smallest node
 // as root; finds shorten the paths they walk (the ECL-CC scheme, Jaiganesh and Burtscher,
-// 2018). Loads and stores of parent are device-coherent: other workgroups change it. ----
+// 2018). The finds' loads may see a stale parent: any value ever stored is an ancestor in
+// the same component, and a hook only succeeds on a node that is a root at the CAS (which
+// returns the current parent otherwise), so they are relaxed atomics of workgroup scope
+// (cacheable); the final flattening runs after the hooking kernel has completed. ----
 __device__ __forceinline__ int32_t cc_load(const int32_t *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 __device__ __forceinline__ void cc_store(int32_t *p, int32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ int32_t cc_find(int32_t *parent, int32_t v) {

@@ -46,8 +46,13 @@ class Operator:
         self.pol = pol
         self.Q = Q
         self.shift = float(shift)
+        use_rec = RECORDS if records is None else bool(records)
+        rec_ok = (pol is not None and precision == "mf64" and use_rec and getattr(A, "images", None) is None
+                  and (lanes is None or int(lanes) // 1000 & 4) and A.records() is not None)
         if lanes is not None:
             self.lanes = int(lanes)
+        elif rec_ok:
+            self.lanes = 6001 if A.clusters() is not None else 161   # plain products only; no tuning needed
         elif tune:
             from .autotune import best_config
             self.lanes = best_config(A, precision)
@@ -70,12 +75,9 @@ class Operator:
             self.mu = [float(x) for x in pol.AdjustedCoefficients[:pol.AdjustedDegree + 1]]
             self.degree = int(pol.AdjustedDegree)
         self.rec = None
-        use_rec = RECORDS if records is None else bool(records)
-        if (pol is not None and precision == "mf64" and use_rec and getattr(A, "images", None) is None
-                and (lanes is None or int(lanes) // 1000 & 4)):
+        if rec_ok:
             self.rec = A.records()
-            if self.rec is not None:
-                self._acc_rec = torch.empty(3 * A.clusters()[0].numel(), dtype=torch.float64, device=dev)
+            self._acc_rec = torch.empty(3 * A.clusters()[0].numel(), dtype=torch.float64, device=dev)
         if Q is not None:
             if Q.dtype != torch.float64 or Q.shape[1] != self.n:
                 raise ValueError("Q must be float64 [m, 3N]")
