@@ -463,7 +463,7 @@ __device__ __forceinline__ double cl_pair_r2(const double *__restrict__ xyz, int
 // more than kMaxUnion row entries, bit 2 = the union spans 2^24 columns or more.
 // Fill pass: entries[cl_ptr[c] + r] = ((j - base[c]) << 8) | mask, ascending j.
 template <bool FILL>
-__global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const double *__restrict__ xyz,
+__global__ void __launch_bounds__(256) k_cl_union(int64_t n, int64_t ncl, const double *__restrict__ xyz,
                                                    const int64_t *__restrict__ rowptr,
                                                    const int32_t *__restrict__ col,
                                                    const int32_t *__restrict__ cluster_atoms,
@@ -475,7 +475,7 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const double *__r
   __shared__ int32_t part[256];
   __shared__ int64_t rb[kClusterAtoms], rl[kClusterAtoms + 1];
   __shared__ int32_t ca[kClusterAtoms];
-  __shared__ int32_t s_min, s_max;
+  __shared__ int32_t s_min, s_max, s_bad;
   const int64_t c = blockIdx.x;
   const int tid = threadIdx.x;
   if (tid == 0) {
@@ -491,6 +491,7 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const double *__r
     rl[kClusterAtoms] = acc;
     s_min = 0;
     s_max = 0;
+    s_bad = 0;
   }
   __syncthreads();
   const int64_t total64 = rl[kClusterAtoms];
@@ -512,11 +513,26 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t ncl, const double *__r
     val[e] = col[rb[t] + (e - rl[t])];
   }
   __syncthreads();
+  // rows out of order would make the ranks collide and leave keys unwritten, and a column
+  // outside [0, n) would address coordinates out of bounds: the block stops here, before any
+  // rank is formed (uniform exit)
+  for (int e = tid; e < total; e += blockDim.x) {
+    int t = 0;
+    while (e >= rl[t + 1]) ++t;
+    if ((e > rl[t] && val[e - 1] >= val[e]) || val[e] < 0 || val[e] >= n) s_bad = 1;
+  }
+  __syncthreads();
+  if (s_bad) {
+    if (tid == 0) {
+      atomicOr(overflow, 4);
+      if (!FILL) counts[c] = 0;
+    }
+    return;
+  }
   for (int e = tid; e < total; e += blockDim.x) {
     int t = 0;
     while (e >= rl[t + 1]) ++t;
     const int32_t j = val[e];
-    if (e > rl[t] && val[e - 1] >= j) atomicOr(overflow, 4);
     int rank = e - (int)rl[t];
     for (int u = 0; u < kClusterAtoms; ++u) {
       if (u == t) continue;
@@ -941,7 +957,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
     HIP_TRY(hipStreamSynchronize(st));
     if (over & 1) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters (%s): a cluster's rows hold more than %d entries", pass, kMaxUnion);
     if (over & 2) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters (%s): a cluster's neighbours span 2^24 columns or more", pass);
-    if (over & 4) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_clusters (%s): a row's columns are not in ascending order", pass);
+    if (over & 4) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_clusters (%s): a row's columns are not in ascending order or outside [0, n)", pass);
     return INCH_OK;
   };
   if (entries) {  // fill pass on the layout of the count pass
@@ -949,7 +965,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
     HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (total > capacity) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters: need %lld > capacity %lld", (long long)total, (long long)capacity);
-    k_cl_union<true><<<(unsigned)ncl, 256, 0, st>>>(ncl, xyz, rowptr, col, cluster_atoms, cl_ptr, base, nullptr, entries,
+    k_cl_union<true><<<(unsigned)ncl, 256, 0, st>>>(n, ncl, xyz, rowptr, col, cluster_atoms, cl_ptr, base, nullptr, entries,
                                                     (int32_t *)ovf.p);
     HIP_TRY(hipGetLastError());
     inch_status s = check_overflow("fill");
@@ -1019,7 +1035,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   HIP_TRY(hipMalloc(&base0.p, ncl * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&perm.p, ncl * sizeof(uint64_t)));
   HIP_TRY(hipMemcpyAsync(atoms0.p, cluster_atoms, slots * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(ncl, xyz, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
+  k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(n, ncl, xyz, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
                                                    (int64_t *)counts0.p, nullptr, (int32_t *)ovf.p);
   HIP_TRY(hipGetLastError());
   {

@@ -60,6 +60,46 @@ def test_neighbours_bit_exact_40k_nm_units(glob40k):
     assert np.array_equal(ix.cpu().numpy(), ix_ref)
 
 
+def test_neighbours_long_rows_and_renumbering(glob2k):
+    """Rows of several hundred entries (rc = 20 on 2,000 atoms: up to ~1,000 neighbours), which
+    the row sort handles by its bitonic path (the rank path stops at 256), bit-exact against
+    the oracle; and the CSR renumbered into an RCM order equals the oracle's CSR of the
+    reordered coordinates."""
+    from inching_amd import ops
+    _print("case: neighbours n=2000 rc=20 (long rows)")
+    ip_ref, ix_ref = OG.neighbour_csr(glob2k, 20.0)
+    assert np.diff(ip_ref).max() > 256
+    ip, ix = ops.neighbours(_dev(glob2k), 20.0)
+    assert np.array_equal(ip.cpu().numpy(), ip_ref)
+    assert np.array_equal(ix.cpu().numpy(), ix_ref)
+    order, undo = ops.rcm_device(ip, ix)
+    rp, rc_ = ops.csr_permute(ip, ix, order, undo)
+    o = order.cpu().numpy()
+    ip2, ix2 = OG.neighbour_csr(glob2k[o], 20.0)
+    assert np.array_equal(rp.cpu().numpy(), ip2) and np.array_equal(rc_.cpu().numpy(), ix2)
+
+
+def test_cluster_layout_rejects_unsorted_rows(glob2k):
+    """The cluster lists merge the atoms' rows, which must be in ascending column order with
+    columns in [0, n): a row out of order or a column out of range is an error
+    (INCH_ERR_INVALID), found before any merge rank or coordinate is formed from it."""
+    from inching_amd import ops
+    from inching_amd._lib import InchingError
+    _print("case: cluster layout, unsorted rows")
+    ip, ix = ops.neighbours(_dev(glob2k), 8.0)
+    bad = ix.clone()
+    b0 = int(ip[5].item())
+    bad[b0], bad[b0 + 1] = ix[b0 + 1], ix[b0]          # swap two columns of row 5
+    xyz = _dev(glob2k)
+    with pytest.raises(InchingError):
+        ops.anm_clusters(xyz, ip, bad, 4.0, 2.0)
+    far = ix.clone()
+    far[int(ip[6].item()) - 1] = len(glob2k) + 5        # last column of row 5 beyond n (still ascending)
+    with pytest.raises(InchingError):
+        ops.anm_clusters(xyz, ip, far, 4.0, 2.0)
+    ops.anm_clusters(xyz, ip, ix, 4.0, 2.0)             # the sorted rows are accepted
+
+
 def test_rcm_pipeline_matches_oracle(glob2k):
     from inching_amd.hessian import rcm_order
     _print("case: rcm pipeline n=2000")
