@@ -13,8 +13,11 @@ and ``IterativeSolvers.py:216-305`` (``gmresDeflate``):
 * the correction is orthogonalised against the locked vectors (MGS) and V (ICGS, <= 3 passes).
 
 The operator is A (+ shift Q^T Q for the Hotelling deflation), float64 stored or
-matrix-free (``Operator`` with precision 'f64' or 'mf64'). Device vectors are rows of
-row-major blocks; Gram products use the multi-dot kernel.
+matrix-free (``Operator`` with precision 'f64' or 'mf64'). The correction solve may run on
+a second operator (``inner``): float32 stored values with float64 vectors, the mixed
+precision form; it only shapes the search direction, so Ritz values, residuals and
+acceptance stay on the float64 operator. Device vectors are rows of row-major blocks;
+Gram products use the multi-dot kernel.
 """
 import time
 
@@ -30,10 +33,13 @@ class JDMResult:
 
 
 class JDM:
-    def __init__(self, op, k, factor=2, restart=20):
+    def __init__(self, op, k, factor=2, restart=20, inner=None):
         if factor <= 1:
             raise AssertionError("ABORTED. User_WorkspaceSizeFactor: int > 1")
         self.op = op
+        self.inner = op if inner is None else inner
+        if self.inner.n != op.n:
+            raise ValueError("the correction operator has %d rows, the operator %d" % (self.inner.n, op.n))
         self.k = int(k)
         self.n = op.n
         self.jmax = self.k * int(factor)
@@ -56,6 +62,9 @@ class JDM:
         self.s1 = torch.zeros(1, **f64)
         self.Hd = torch.zeros((self.restart, self.restart + 1), **f64)   # row j = column j of H
         self.ws = ops.Workspace(self.h.numel(), dev)
+
+    def _n_spmv(self):
+        return self.op.n_spmv + (self.inner.n_spmv if self.inner is not self.op else 0)
 
     # -- small device helpers ------------------------------------------------
     def _dot(self, a, b):
@@ -102,7 +111,7 @@ class JDM:
         while True:
             mx.copy_(x)
             self._project(uu, mx)
-            self.op.apply(mx, self.w)
+            self.inner.apply(mx, self.w)
             ops.axpby(-sigma, mx, 1.0, self.w)
             self._project(uu, self.w)
             # w <- b - w
@@ -118,7 +127,7 @@ class JDM:
                 z = self.z
                 z.copy_(self.Vg[j])
                 self._project(uu, z)
-                self.op.apply(z, self.Az)
+                self.inner.apply(z, self.Az)
                 ops.axpby(-sigma, z, 1.0, self.Az)
                 # the reference subtracts (uu . z) uu here, which is 0 after the projection
                 ops.multidot(self.Vg, j + 1, self.Az, Hd[j, :j + 1], self.ws)
@@ -197,7 +206,7 @@ class JDM:
                     idx = np.argsort(eig)
                     torch.cuda.synchronize()
                     return JDMResult(eigval=eig[idx], eigvec=Qa[torch.as_tensor(idx, device=V.device)].clone(),
-                                     iters=it, converged=nconv, n_spmv=self.op.n_spmv, seconds=time.time() - t0)
+                                     iters=it, converged=nconv, n_spmv=self._n_spmv(), seconds=time.time() - t0)
             if S.shape[0] == self.jmax:
                 self._combine(V, W.shape[0], W[:, :self.jmin], self.Vtmp)
                 V[:self.jmin].copy_(self.Vtmp[:self.jmin])
@@ -233,4 +242,4 @@ class JDM:
         idx = np.argsort(eig)
         torch.cuda.synchronize()
         return JDMResult(eigval=eig[idx], eigvec=Qa[torch.as_tensor(idx[:self.jmin], device=V.device)].clone(),
-                         iters=it, converged=nconv, n_spmv=self.op.n_spmv, seconds=time.time() - t0)
+                         iters=it, converged=nconv, n_spmv=self._n_spmv(), seconds=time.time() - t0)

@@ -244,25 +244,33 @@ def lowest_modes_jdm(A, k, Q=None, precision="mf64", tol_rel=1e-8, V0=None, shif
     search space V0 (rows). Accepted on the float64 residual after a float64 Rayleigh-Ritz
     polish, like ``lowest_modes``."""
     from .jdm import JDM
+    if precision not in ("f64", "mf64", "mixed"):
+        raise ValueError("precision must be 'f64', 'mf64' or 'mixed'")
     t0 = time.time()
     n_spmv = {}
+    mixed = precision == "mixed"
     if bound is None:
         binfo = {}
-        lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL, info=binfo)
+        lo, hi = CH.spectrum_bound(A, precision="f32" if mixed else precision, stop_rel=CH.BOUND_STOP_REL,
+                                   info=binfo)
         bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
         n_spmv["bound"] = binfo["products"]
     lam_max = bound[1]
-    op = Operator(A, precision, Q=Q, shift=shift)
+    op = Operator(A, "mf64" if mixed else precision, Q=Q, shift=shift)
+    inner = Operator(A, "f32", Q=Q, shift=shift) if mixed else None
     t = time.time()
-    r = JDM(op, k, factor=factor).run(tol=0.5 * tol_rel * lam_max, maxiter=maxiter, V0=V0, seed=seed)
+    r = JDM(op, k, factor=factor, inner=inner).run(tol=0.5 * tol_rel * lam_max, maxiter=maxiter, V0=V0, seed=seed)
     phases = {"solve_jdm_" + precision: time.time() - t}
     n_spmv["jdm"] = op.n_spmv
+    if mixed:
+        n_spmv["jdm_correction_f32"] = inner.n_spmv
     theta, X, info = polish(A, r.eigvec, Q=Q, lam_max=lam_max)
     state = {"last": (theta, X, info["rel_res"])}
     theta, X, rel, converged = _final(A, Q, lam_max, tol_rel, k, X, state, "JDM", int(r.iters))
     torch.cuda.synchronize()
     return ModeResult(eigval=np.asarray(theta), eigvec=X, rel_res=np.asarray(rel), lam_max=lam_max,
-                      phases=phases, n_spmv=n_spmv, total_spmv=op.n_spmv, restarts=int(r.iters), n_apply=op.n_apply,
+                      phases=phases, n_spmv=n_spmv, total_spmv=r.n_spmv, restarts=int(r.iters),
+                      n_apply=op.n_apply + (inner.n_apply if mixed else 0),
                       ncv=factor * k, degree=0, seconds=time.time() - t0, precision=precision,
                       n_converged=int(r.converged), converged=converged and int(r.converged) >= k)
 
@@ -273,7 +281,7 @@ def lowest_modes_jdm_warm(A, k, net, Q=None, precision="mf64", tol_rel=1e-8, **k
     tolerance, so the returned modes never depend on the network."""
     from .warmstart import warm_block
     t0 = time.time()
-    lo, hi = CH.spectrum_bound(A, precision=precision, stop_rel=CH.BOUND_STOP_REL)
+    lo, hi = CH.spectrum_bound(A, precision="f32" if precision == "mixed" else precision, stop_rel=CH.BOUND_STOP_REL)
     bound = (A.plus_i, hi * (1 + 1e-3) + 1e-12)
     theta, X, info = warm_block(net, A, Q=Q, lam_max=bound[1])
     if not _usable_block(theta, info):

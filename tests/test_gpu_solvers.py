@@ -139,6 +139,24 @@ def test_mixed_precision_filtered_kpm(problem):
     _check(Ad, w, U, r.eigval, V, k, 1e-8, 1e-8)
 
 
+def test_jdm_mixed_precision(problem):
+    """Jacobi-Davidson with the correction solves on float32 stored values (float64 vectors)
+    and the outer iteration, residuals and acceptance on the float64 matrix-free operator:
+    the same modes as dense eigh to the float64 bars, the float32 operator used, and the
+    eigenvalues equal to the all-float64 JDM's."""
+    from inching_amd.pipeline import lowest_modes_jdm
+    A, Ad, w, U, Q = problem
+    print("case: lowest_modes_jdm mixed k=12", flush=True)
+    k = 12
+    r = lowest_modes_jdm(A, k, Q=Q, precision="mixed", tol_rel=1e-8)
+    assert r.converged and np.max(r.rel_res) <= 1e-8
+    V = r.eigvec.T.cpu().numpy()
+    _check(Ad, w, U, r.eigval, V, k, 1e-8, 1e-8)
+    assert r.n_spmv["jdm_correction_f32"] > 0 and r.total_spmv == r.n_spmv["jdm"] + r.n_spmv["jdm_correction_f32"]
+    ref = lowest_modes_jdm(A, k, Q=Q, precision="mf64", tol_rel=1e-8)
+    assert np.max(np.abs(r.eigval - ref.eigval)) <= 1e-9
+
+
 def test_trlm_filtered(problem):
     from inching_amd.chebyshev import design_filter
     from inching_amd.operator import Operator
