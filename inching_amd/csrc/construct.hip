@@ -637,7 +637,8 @@ __global__ void k_rec_coords(int64_t slots, const int32_t *__restrict__ cluster_
   if (at >= 0) slot_of[at] = (int32_t)q;
 }
 
-// One block per cluster: its list rewritten as (slot of j) << 4 | mask and sorted ascending
+// One block per cluster: its list rewritten as 48 (slot of j) | mask -- the byte offset of
+// j's 48-byte record, whose low 4 bits are zero, carrying the mask -- and sorted ascending
 // (bitonic sort in LDS; a list holds fewer than kMaxUnion entries).
 __global__ void __launch_bounds__(256) k_rec_lists(int64_t ncl, const int64_t *__restrict__ cl_ptr,
                                                     const int32_t *__restrict__ base,
@@ -654,7 +655,7 @@ __global__ void __launch_bounds__(256) k_rec_lists(int64_t ncl, const int64_t *_
     uint32_t k = 0xffffffffu;
     if (e < len) {
       const uint32_t w = entries[e0 + e];
-      k = ((uint32_t)slot_of[b0 + (w >> 8)] << 4) | (w & 15u);
+      k = 48u * (uint32_t)slot_of[b0 + (w >> 8)] | (w & 15u);
     }
     key[e] = k;
   }

@@ -259,7 +259,8 @@ def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None):
 def anm_records(xyz, clusters, copies=3, stream=None):
     """Record form of the cluster layout (inch_anm_records): (list of ``copies`` record
     buffers float64 [24 ncl], each with the slots' coordinates and a zero vector part,
-    rec_entries int32 [nnz] holding the uint32 words (slot << 4) | mask)."""
+    rec_entries int32 [nnz] holding the uint32 words 48 slot | mask: the byte offset of the
+    neighbour's 48-byte record with the mask in its low 4 bits)."""
     require_gpu()
     _check_dev(xyz)
     n = xyz.shape[0]

@@ -244,8 +244,9 @@ inch_status inch_anm_matfree_cl_apply(int64_t n, const double *xyz, int64_t ncl,
  * recurrence vectors of ChebAv, Burn/PolynomialFilters/T2.py:28-92, around the product of
  * Burn/Krylov/T3.py:54-239). Slot q = 4 c + t is atom t of cluster c. inch_anm_records
  * writes rec float64 [24 ncl] (slot q: coordinates at 6q..6q+2, zeros for a padding slot;
- * vector part 6q+3..6q+5 zeroed) and rec_entries uint32 [nnz of the layout] = (slot of j)
- * << 4 | mask per cluster, ascending slot (cl_ptr unchanged). INCH_ERR_CAPACITY when
+ * vector part 6q+3..6q+5 zeroed) and rec_entries uint32 [nnz of the layout] = 48 (slot of
+ * j) | mask per cluster (the byte offset of j's record, low 4 bits holding the mask),
+ * ascending slot (cl_ptr unchanged). INCH_ERR_CAPACITY when
  * 192 ncl exceeds 2^32 - 1 (32-bit record offsets). Syncs. */
 inch_status inch_anm_records(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,
                              const int64_t *cl_ptr, const int32_t *base, const uint32_t *entries, double *rec,

@@ -642,8 +642,9 @@ def test_record_layout_and_product(mat, case):
     w = rent.cpu().numpy().view(np.uint32).astype(np.int64)
     for c, (j, m) in enumerate(lists):
         e = w[ptr[c]:ptr[c + 1]]
-        assert np.all(np.diff(e >> 4) > 0)
-        got = sorted(zip((e >> 4).tolist(), (e & 15).tolist()))
+        off = e & ~15                      # the record's byte offset, 48 bytes per slot
+        assert np.all(off % 48 == 0) and np.all(np.diff(off) > 0)
+        got = sorted(zip((off // 48).tolist(), (e & 15).tolist()))
         assert got == sorted(zip(slot_of[j].tolist(), m.tolist())), c
     # one step with every epilogue term, against the cluster kernel in the matrix's order
     nd = A.shape[0]
