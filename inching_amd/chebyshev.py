@@ -437,8 +437,9 @@ class KPMCounter:
 
 
 # early end of the bound run for callers that use only its maximum (which they widen by
-# 1e-3): on the 2.4M-atom assembly 1e-5 stops after 81 products instead of 111 (1e-10), with
-# the maximum 1.3e-7 relative below that of 400 steps
+# 1e-3): on the 2.4M-atom assembly 1e-5 stops after 71 products instead of 101 (1e-10), with
+# the maximum 2.2e-8 relative below that of 400 steps. A looser 1e-4 is not safe: on a
+# 1,200-atom globule it stopped 2.3e-3 below the largest eigenvalue, outside the widening
 BOUND_STOP_REL = 1e-5
 BOUND_CHECK_EVERY = 10
 
