@@ -27,6 +27,12 @@ BSR_PREFERENCE = tuple(1200 + g for g in (32, 64, 16, 8, 4)) + tuple(200 + g for
 CLUSTER_PREFERENCE = (6001, 6104, 6002, 6108, 6004, 6116, 4001, 4104, 4002, 4108, 4004, 4116)
 MATFREE_CONFIGS = (41, 42, 81, 82, 161, 162, 321, 322)
 CLUSTER_CONFIGS = (1, 2, 4, 104, 108, 116)   # list entries per lane iteration; 100 + clusters per wave (persistent)
+# From this many atoms on, a matrix with the cluster layout times only the cluster kernels:
+# their best shape was faster than the best per-row shape by 1.22x (30k-atom globule), 1.50x
+# (300k), 1.39x (512-globule batch, 2.7M atoms) and 1.45x (2.4M-atom assembly), while the
+# per-row shapes won below (0.85x at 1,200 atoms, 0.94x at 5,000). Timing the 32 per-row
+# shapes of a 2.7M-atom batch took about 0.35 s.
+CLUSTER_ONLY_ATOMS = 20000
 
 _cache = {}
 _loaded = False
@@ -94,9 +100,12 @@ def best_config(A, precision, reps=5):
     if kind == "matfree":
         # lanes*10+unroll, + 1000 for 16-bit column offsets, + 2000 for XCD-contiguous rows,
         # + 4000 for the cluster layout
-        flags = (0, 2000) + ((1000, 3000) if A.col16() is not None else ())
-        codes = tuple(f + c for f in flags for c in MATFREE_CONFIGS)
-        if A.clusters() is not None:
+        clusters = A.clusters() is not None
+        codes = ()
+        if not clusters or A.n_atoms < CLUSTER_ONLY_ATOMS:
+            flags = (0, 2000) + ((1000, 3000) if A.col16() is not None else ())
+            codes = tuple(f + c for f in flags for c in MATFREE_CONFIGS)
+        if clusters:
             codes += tuple(f + c for f in (4000, 6000) for c in CLUSTER_CONFIGS)
         for cfg in codes:
             times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
