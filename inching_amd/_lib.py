@@ -73,6 +73,8 @@ SIGNATURES = {
     "inch_seg_update": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, _dbl, _vp, _vp, _vp,
                                   _vp]),
     "inch_seg_scale": (_STATUS, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "inch_seg_rigid_apply": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _vp, _vp,
+                                       _vp]),
     "inch_seg_cgs_b": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     "inch_seg_dcgs2_work_size": (_i64, [_i32, _i32, _i32]),
     "inch_seg_dcgs2_step": (_STATUS, [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),

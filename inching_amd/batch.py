@@ -98,6 +98,14 @@ class SegOps:
              P(V) if J > 0 else None, V.stride(0) if J > 0 else u.numel(), P(h) if J > 0 else None, int(hld),
              float(scale), P(u), P(nrm), P(self.wk) if nrm is not None else None, SP())
 
+    def rigid(self, rig, x, y, scale):
+        """y -= scale * P x, P the per-segment rigid-body projection of ``rig`` (a RigidBody of
+        the same segments); x may be y. With y None only the moments of x are formed, into
+        rig.g [S, 6] (sum of x's components, sum r_i x x_i with r_i about rig.cen)."""
+        s = self.seg
+        call("inch_seg_rigid_apply", s.nchunk, s.S, P(s.cbeg), P(s.cend), P(s.cseg), P(s.sptr), P(rig.xyz),
+             P(rig.cen), P(rig.kinv), P(rig.ninv), P(x), P(y), float(scale), P(rig.g), P(self.wk), SP())
+
     def scale(self, src, div, dst, dst2=None):
         s = self.seg
         call("inch_seg_scale", s.nchunk, P(s.cbeg), P(s.cend), P(s.cseg), P(src), P(div), P(dst), P(dst2), SP())
@@ -159,6 +167,51 @@ def segmented_rigid_basis(xyz, seg_atoms, segops):
         segops.combine(Q, 6, W, Qn)
         Q = Qn
     return Q
+
+
+class RigidBody:
+    """Per-segment data of the rigid-body projection computed from the coordinates
+    (``SegOps.rigid``): centroids, inverse rotational Gram matrices K_s^{-1} =
+    (sum_i |r_i|^2 I - r_i r_i^T)^{-1} and inverse atom counts. The projection equals
+    x -> Q^T Q x for the orthonormal rows of ``segmented_rigid_basis`` and reads the
+    coordinates (24 bytes per atom) instead of six basis rows (144 bytes per atom).
+
+    The centroids and K_s come from the same segmented moment kernel (no atomics: a scatter-add
+    of every atom into its segment's nine entries took seconds for 512 segments): the moments
+    of the coordinates about the origin give the centroids, and those of the rotation field
+    e_l x r_i give K_s e_l = sum_i r_i x (e_l x r_i)."""
+
+    def __init__(self, xyz, seg_atoms, so):
+        dev = xyz.device
+        seg_atoms = np.asarray(seg_atoms, dtype=np.int64)
+        S = len(seg_atoms) - 1
+        cnt = np.diff(seg_atoms)
+        if np.any(cnt < 3):
+            raise ValueError("every segment needs at least 3 atoms for its rigid-body basis")
+        if so.seg.S != S or so.seg.nchunk == 0 or int(so.seg.seg_ptr[-1]) != 3 * int(seg_atoms[-1]):
+            raise ValueError("the segment operations must cover the atoms' dofs")
+        self.xyz = xyz.contiguous()
+        self.g = torch.empty(S * 6, dtype=torch.float64, device=dev)
+        self.kinv = self.ninv = None
+        self.cen = torch.zeros((S, 3), dtype=torch.float64, device=dev)
+        so.rigid(self, self.xyz.reshape(-1), None, 0.0)            # moments about the origin
+        cnt_d = torch.as_tensor(cnt.astype(np.float64), device=dev)
+        self.cen = (self.g.view(S, 6)[:, :3] / cnt_d[:, None]).contiguous()
+        comp = torch.repeat_interleave(torch.arange(S, device=dev), torch.as_tensor(cnt, device=dev))
+        r = self.xyz - self.cen[comp]
+        zero = torch.zeros_like(r[:, 0])
+        K = torch.empty((S, 3, 3), dtype=torch.float64, device=dev)
+        for l, field in enumerate(((zero, -r[:, 2], r[:, 1]), (r[:, 2], zero, -r[:, 0]), (-r[:, 1], r[:, 0], zero))):
+            so.rigid(self, torch.stack(field, 1).reshape(-1), None, 0.0)
+            K[:, :, l] = self.g.view(S, 6)[:, 3:]
+        # the S 3 x 3 inverses on the host: the device's batched small-matrix routines run
+        # one matrix at a time
+        Kh = K.cpu().numpy()
+        Kh = 0.5 * (Kh + Kh.transpose(0, 2, 1))
+        if np.any(np.abs(np.linalg.det(Kh)) <= 1e-12 * np.linalg.norm(Kh, axis=(1, 2)) ** 3):
+            raise ValueError("a segment's atoms are collinear: its rotations do not span three directions")
+        self.kinv = torch.as_tensor(np.linalg.inv(Kh).reshape(S, 9), device=dev)
+        self.ninv = torch.as_tensor(1.0 / cnt.astype(np.float64), device=dev)
 
 
 class BatchResult:
@@ -334,7 +387,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
         raise AssertionError("ABORTED. k must be smaller than every structure's dof count minus the rigid modes")
     so = SegOps(seg, max(ncv, 6) + 1, dev)
     dcgs2 = bool(dcgs2) and ncv <= so.dJ
-    Q = segmented_rigid_basis(A.xyz, seg_atoms, so)
+    rig = RigidBody(A.xyz, seg_atoms, so)
     mixed = precision == "mixed"
     op = Operator(A, "f32" if mixed else precision, pol=pol, timer=timer)
     if timer is not None:
@@ -345,7 +398,6 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     u = torch.empty(n, **f64)
     h = torch.empty(S * (ncv + 1), **f64)
     h2 = torch.empty(S * (ncv + 1), **f64)
-    hq = torch.empty(S * 6, **f64)
     alpha = torch.zeros((ncv, S), **f64)
     beta = torch.zeros((ncv, S), **f64)
     scal = torch.empty(S, **f64)
@@ -362,11 +414,9 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     def apply(x, y):
         op.apply(x, y)
         if project:
-            so.multidot(Q, 6, y, hq)
-            so.update(Q, 6, hq, 6, 1.0, y)
+            so.rigid(rig, y, y, 1.0)
         elif shift:
-            so.multidot(Q, 6, x, hq)
-            so.update(Q, 6, hq, 6, -shift, y)
+            so.rigid(rig, x, y, -shift)
 
     def lanczos(i0, i1):
         for i in range(i0, i1):
@@ -502,8 +552,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     else:
         u.copy_(torch.as_tensor(v0, dtype=torch.float64, device=dev).reshape(-1))
     if project:
-        so.multidot(Q, 6, u, hq)
-        so.update(Q, 6, hq, 6, 1.0, u)
+        so.rigid(rig, u, u, 1.0)
     so.update(V, 0, None, 1, 0.0, u, nrm=scal)
     so.scale(u, scal, V[0])
     done_at = np.full(S, np.nan)
@@ -550,7 +599,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
     if mixed:
         phase1_cycles = cycles
         if filtered:
-            checked = _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, verbose=verbose,
+            checked = _fp64_block_refine(A, V, k, pol, rig, so, S, tol_rel, checked, verify, verbose=verbose,
                                          timer=timer, band=band)
         else:
             # no filter to refine with: a float64 lockstep run started from a combination of
@@ -596,7 +645,7 @@ def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1
                        operator_launches=op.n_spmv, phase1_cycles=phase1_cycles, in_band=in_band[order])
 
 
-def _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, maxiter=4, verbose=False, timer=None,
+def _fp64_block_refine(A, V, k, pol, rig, so, S, tol_rel, checked, verify, maxiter=4, verbose=False, timer=None,
                        band=None):
     """Float64 refinement of the Ritz block V[:k] of a float32 phase: filtered subspace steps
     X <- p(X) with the matrix-free float64 operator (the rigid-body rows projected out per
@@ -611,7 +660,6 @@ def _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, maxiter
     n = V.shape[1]
     X = V[:k]
     Y = torch.empty((k, n), **f64)
-    hq = torch.empty(S * 6, **f64)
     h = torch.empty(S * k, **f64)
     theta_h, rel = checked
     for it in range(maxiter):
@@ -621,8 +669,7 @@ def _fp64_block_refine(A, V, k, pol, Q, so, S, tol_rel, checked, verify, maxiter
         for q in range(k):
             op.apply(X[q], Y[q])
             for _ in range(2):
-                so.multidot(Q, 6, Y[q], hq)
-                so.update(Q, 6, hq, 6, 1.0, Y[q])
+                so.rigid(rig, Y[q], Y[q], 1.0)
         Z = A.matfree(Y.T.contiguous()).T.contiguous()
         G = np.empty((S, k, k))
         H = np.empty((S, k, k))

@@ -86,6 +86,79 @@ __global__ void __launch_bounds__(256) k_seg_update(const int64_t *__restrict__ 
   }
 }
 
+// Rigid-body projection per segment from the coordinates instead of six stored basis rows.
+// With r_i = p_i - cen[s] (atom i of segment s relative to the segment's centroid), the
+// translations e_a and the rotations e_k x r_i of the segment are orthogonal to each other
+// (sum_i r_i = 0), the translations have Gram matrix n_s I and the rotations
+// K_s = sum_i (|r_i|^2 I - r_i r_i^T). So the projection of x onto their span is, at atom i,
+// mean_s(x) + (K_s^{-1} sum_j r_j x x_j) x r_i: six moments per segment, then one pass.
+// part[c*6 + a] = sum of component a of x over chunk c (a < 3), part[c*6 + 3 + a] = component
+// a of sum r_i x x_i over the chunk.
+__global__ void __launch_bounds__(256) k_seg_rigid_moments(const int64_t *__restrict__ cbeg,
+                                                            const int64_t *__restrict__ cend,
+                                                            const int32_t *__restrict__ cseg,
+                                                            const double *__restrict__ xyz,
+                                                            const double *__restrict__ cen,
+                                                            const double *__restrict__ x, double *__restrict__ part) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  const double q0 = cen[3 * s], q1 = cen[3 * s + 1], q2 = cen[3 * s + 2];
+  double m[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int64_t d = cbeg[c] + threadIdx.x; d < cend[c]; d += blockDim.x) {
+    const int64_t at = d / 3;
+    const int a = (int)(d - 3 * at);
+    const double v = x[d];
+    const double r0 = xyz[3 * at] - q0, r1 = xyz[3 * at + 1] - q1, r2 = xyz[3 * at + 2] - q2;
+    if (a == 0) {            // r x (v e_x) = v (0, r2, -r1)
+      m[0] += v;
+      m[4] += r2 * v;
+      m[5] -= r1 * v;
+    } else if (a == 1) {     // r x (v e_y) = v (-r2, 0, r0)
+      m[1] += v;
+      m[3] -= r2 * v;
+      m[5] += r0 * v;
+    } else {                 // r x (v e_z) = v (r1, -r0, 0)
+      m[2] += v;
+      m[3] += r1 * v;
+      m[4] -= r0 * v;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    const double tot = block_sum(m[t], red);
+    if (threadIdx.x == 0) part[(int64_t)c * 6 + t] = tot;
+  }
+}
+
+// y -= scale * P x over chunk c of segment s: g[s*6..] = the segment's moments of x,
+// kinv[s*9..] = K_s^{-1} (row-major), ninv[s] = 1 / n_s.
+__global__ void __launch_bounds__(256) k_seg_rigid_update(const int64_t *__restrict__ cbeg,
+                                                           const int64_t *__restrict__ cend,
+                                                           const int32_t *__restrict__ cseg,
+                                                           const double *__restrict__ xyz,
+                                                           const double *__restrict__ cen,
+                                                           const double *__restrict__ kinv,
+                                                           const double *__restrict__ ninv,
+                                                           const double *__restrict__ g, double scale,
+                                                           double *__restrict__ y) {
+  const int c = blockIdx.x;
+  const int s = cseg[c];
+  const double *gs = g + 6 * (int64_t)s, *K = kinv + 9 * (int64_t)s;
+  const double t0 = gs[0] * ninv[s], t1 = gs[1] * ninv[s], t2 = gs[2] * ninv[s];
+  const double f0 = K[0] * gs[3] + K[1] * gs[4] + K[2] * gs[5];
+  const double f1 = K[3] * gs[3] + K[4] * gs[4] + K[5] * gs[5];
+  const double f2 = K[6] * gs[3] + K[7] * gs[4] + K[8] * gs[5];
+  const double q0 = cen[3 * s], q1 = cen[3 * s + 1], q2 = cen[3 * s + 2];
+  for (int64_t d = cbeg[c] + threadIdx.x; d < cend[c]; d += blockDim.x) {
+    const int64_t at = d / 3;
+    const int a = (int)(d - 3 * at);
+    const double r0 = xyz[3 * at] - q0, r1 = xyz[3 * at + 1] - q1, r2 = xyz[3 * at + 2] - q2;
+    const double w = a == 0 ? t0 + (f1 * r2 - f2 * r1) : (a == 1 ? t1 + (f2 * r0 - f0 * r2) : t2 + (f0 * r1 - f1 * r0));
+    y[d] -= scale * w;
+  }
+}
+
 // The update and the second projection of CGS2 in one read of V, per segment:
 // u' = u - V[:J] h (h of the chunk's segment, row stride hld) and, per chunk c,
 // part[c*(J+1) + j] = V[j, chunk] . u'[chunk] (j < J), part[c*(J+1) + J] = |u'[chunk]|^2.
@@ -380,6 +453,24 @@ inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, cons
     k_seg_reduce<<<(unsigned)((S + 255) / 256), 256, 0, st>>>(S, 1, sptr, work, nrm_out, 1);
     HIP_TRY(hipGetLastError());
   }
+  return INCH_OK;
+}
+
+inch_status inch_seg_rigid_apply(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend,
+                                 const int32_t *cseg, const int64_t *sptr, const double *xyz, const double *cen,
+                                 const double *kinv, const double *ninv, const double *x, double *y, double scale,
+                                 double *g, double *work, void *stream) {
+  INCH_REQUIRE(nchunk > 0 && S > 0 && cbeg && cend && cseg && sptr && xyz && cen && x && g && work,
+               "inch_seg_rigid_apply: bad args");
+  INCH_REQUIRE(!y || (kinv && ninv), "inch_seg_rigid_apply: the update needs kinv and ninv");
+  hipStream_t st = (hipStream_t)stream;
+  k_seg_rigid_moments<<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, xyz, cen, x, work);
+  HIP_TRY(hipGetLastError());
+  k_seg_reduce<<<(unsigned)((6 * (int64_t)S + 255) / 256), 256, 0, st>>>(S, 6, sptr, work, g, 0);
+  HIP_TRY(hipGetLastError());
+  if (!y) return INCH_OK;
+  k_seg_rigid_update<<<nchunk, 256, 0, st>>>(cbeg, cend, cseg, xyz, cen, kinv, ninv, g, scale, y);
+  HIP_TRY(hipGetLastError());
   return INCH_OK;
 }
 

@@ -371,6 +371,20 @@ inch_status inch_seg_multidot(int32_t nchunk, int32_t S, const int64_t *cbeg, co
 inch_status inch_seg_update(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend, const int32_t *cseg,
                             const int64_t *sptr, int32_t J, const double *V, int64_t ld, const double *h, int32_t hld,
                             double scale, double *u, double *nrm_out, double *work, void *stream);
+/* Rigid-body (Hotelling) projection per segment from the coordinates (replaces the products
+ * with the six orthonormal rigid-body rows of the driver's deflation basis,
+ * commandBenchmark_InchingTRLMHD...:314-372, used by
+ * Burn/ThickRestartLanczosHotellingDeflation/T1.py:465-520): y -= scale * P x, where at atom i
+ * of segment s, P x = mean_s(x) + (K_s^{-1} sum_j r_j x x_j) x r_i with r_i = xyz_i - cen[s],
+ * K_s = sum_i (|r_i|^2 I - r_i r_i^T). xyz float64 [n, 3] in the vector's atom order (dof 3i+a
+ * is component a of atom i), cen [S, 3] the segments' centroids, kinv [S, 9] row-major
+ * K_s^{-1}, ninv [S] = 1 / atoms of s. x and y [3n] may be the same vector (y <- y - scale P y).
+ * g receives the S x 6 moments (sum of x's components, sum r_i x x_i); with y NULL only they
+ * are formed (kinv and ninv may then be NULL). work >= nchunk * 6 doubles. */
+inch_status inch_seg_rigid_apply(int32_t nchunk, int32_t S, const int64_t *cbeg, const int64_t *cend,
+                                 const int32_t *cseg, const int64_t *sptr, const double *xyz, const double *cen,
+                                 const double *kinv, const double *ninv, const double *x, double *y, double scale,
+                                 double *g, double *work, void *stream);
 /* CGS2's update and second projection in one read of V, per segment: u -= V[:J] h (h of
  * each segment, row stride hld), then h2[s, j] = V[j] . u over segment s (j < J),
  * h2[s, J] = |u|^2 over s and nrm_out[s] = sqrt(h2[s, J]); h2 is [S, J + 1]. work holds

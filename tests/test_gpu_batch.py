@@ -62,6 +62,66 @@ def test_batched_segmented_kernels(cuda):
             assert np.allclose(out.cpu().numpy()[:, sl], W[s].T @ V[:, sl], rtol=1e-11, atol=1e-10), (kk, s)
 
 
+def test_rigid_projection_from_coordinates(cuda):
+    """SegOps.rigid (the per-segment rigid-body projection from the coordinates) against the
+    projection onto the QR basis of each segment's six raw rigid-body fields (numpy) and the
+    stored orthonormal rows of segmented_rigid_basis: y -= scale P x out of place and in
+    place, chunks that split atoms, segments far from the origin and a 3-atom segment; rigid
+    motions are removed exactly by I - P."""
+    import torch
+    from inching_amd.batch import RigidBody, SegOps, Segments, place_apart, segmented_rigid_basis
+    rng = np.random.default_rng(8)
+    structs = [small_globule(700, seed=81), rng.standard_normal((3, 3)) * 3.0, small_globule(2000, seed=82),
+               small_globule(1500, seed=83)]
+    X, seg_atoms = place_apart(structs)
+    X = X + np.array([500.0, -300.0, 800.0])
+    dev = torch.device("cuda")
+    xyz = torch.as_tensor(X, device=dev)
+    S = len(structs)
+    seg = Segments(3 * seg_atoms, dev, chunk=1024)             # 1024 dofs: chunks split atoms
+    so = SegOps(seg, 8, dev)
+    rig = RigidBody(xyz, seg_atoms, so)
+    n3 = 3 * X.shape[0]
+    x, y0 = rng.standard_normal(n3), rng.standard_normal(n3)
+    ref = y0.copy()
+    for s in range(S):
+        a0, a1 = seg_atoms[s], seg_atoms[s + 1]
+        p = X[a0:a1]
+        B = np.zeros((6, a1 - a0, 3))
+        for c in range(3):
+            B[c, :, c] = 1.0
+            e = np.zeros(3)
+            e[c] = 1.0
+            B[3 + c] = np.cross(e, p)
+        Qs, _ = np.linalg.qr(B.reshape(6, -1).T)
+        xs = x[3 * a0:3 * a1]
+        ref[3 * a0:3 * a1] -= 0.7 * (Qs @ (Qs.T @ xs))
+    yd = torch.as_tensor(y0, device=dev)
+    so.rigid(rig, torch.as_tensor(x, device=dev), yd, 0.7)
+    np.testing.assert_allclose(yd.cpu().numpy(), ref, rtol=0, atol=1e-11 * np.abs(ref).max())
+    Q = segmented_rigid_basis(xyz, seg_atoms, so)
+    hq = torch.empty(S * 6, dtype=torch.float64, device=dev)
+    yq = torch.as_tensor(y0, device=dev)
+    so.multidot(Q, 6, torch.as_tensor(x, device=dev), hq)
+    so.update(Q, 6, hq, 6, 0.7, yq)
+    np.testing.assert_allclose(yd.cpu().numpy(), yq.cpu().numpy(), rtol=0, atol=1e-11 * np.abs(ref).max())
+    # in place: (I - P) x; a rigid motion of every segment is removed
+    xi = torch.as_tensor(x, device=dev)
+    so.rigid(rig, xi, xi, 1.0)
+    xq = torch.as_tensor(x, device=dev)
+    so.multidot(Q, 6, xq, hq)
+    so.update(Q, 6, hq, 6, 1.0, xq)
+    np.testing.assert_allclose(xi.cpu().numpy(), xq.cpu().numpy(), rtol=0, atol=1e-11 * np.abs(x).max())
+    motion = np.zeros((X.shape[0], 3))
+    for s in range(S):
+        a0, a1 = seg_atoms[s], seg_atoms[s + 1]
+        t, w = rng.standard_normal(3), rng.standard_normal(3) * 0.01
+        motion[a0:a1] = t + np.cross(w, X[a0:a1])
+    m = torch.as_tensor(motion.reshape(-1), device=dev)
+    so.rigid(rig, m, m, 1.0)
+    assert np.abs(m.cpu().numpy()).max() < 1e-10 * np.abs(motion).max()
+
+
 @pytest.mark.parametrize("J", [11, 70, 150])
 def test_segmented_fused_cgs_pass(cuda, J):
     """u -= V h per segment, second projection and norm in one pass (three tile widths)."""
