@@ -31,7 +31,9 @@ CLUSTER_CONFIGS = (1, 2, 4, 104, 108, 116)   # list entries per lane iteration; 
 # their best shape was faster than the best per-row shape by 1.22x (30k-atom globule), 1.50x
 # (300k), 1.39x (512-globule batch, 2.7M atoms) and 1.45x (2.4M-atom assembly), while the
 # per-row shapes won below (0.85x at 1,200 atoms, 0.94x at 5,000). Timing the 32 per-row
-# shapes of a 2.7M-atom batch took about 0.35 s.
+# shapes of a 2.7M-atom batch took about 0.35 s. Such matrices skip the persistent cluster
+# shapes too, which lost to the plain ones there (0.101 vs 0.092 ms at 300k atoms, 0.894 vs
+# 0.815 ms on the assembly) and took 47 ms of tuning on the batch.
 CLUSTER_ONLY_ATOMS = 20000
 
 _cache = {}
@@ -101,12 +103,13 @@ def best_config(A, precision, reps=5):
         # lanes*10+unroll, + 1000 for 16-bit column offsets, + 2000 for XCD-contiguous rows,
         # + 4000 for the cluster layout
         clusters = A.clusters() is not None
+        large = A.n_atoms >= CLUSTER_ONLY_ATOMS
         codes = ()
-        if not clusters or A.n_atoms < CLUSTER_ONLY_ATOMS:
+        if not clusters or not large:
             flags = (0, 2000) + ((1000, 3000) if A.col16() is not None else ())
             codes = tuple(f + c for f in flags for c in MATFREE_CONFIGS)
         if clusters:
-            codes += tuple(f + c for f in (4000, 6000) for c in CLUSTER_CONFIGS)
+            codes += tuple(f + c for f in (4000, 6000) for c in CLUSTER_CONFIGS if not (large and c > 100))
         for cfg in codes:
             times[cfg] = _time(lambda: A.matfree_spmv(x, y, s1=0.1, c0=1.0, z=z, c2=1.0, acc=acc, mu=0.1, config=cfg),
                                reps)
