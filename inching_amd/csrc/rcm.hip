@@ -185,23 +185,38 @@ __device__ int32_t cc_find(int32_t *parent, int32_t v) {
   return par;
 }
 
+// Lanes per node of the two edge passes: a node's row is read by kCcLanes consecutive lanes
+// (coalesced), not by one lane walking it alone.
+constexpr int kCcLanes = 16;
+
 // parent[v] = v's smallest neighbour when that is smaller than v (a valid hook: parent[v] <=
 // v and in v's component), which leaves most trees shallow before any CAS
 __global__ void k_cc_init(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
                           int32_t *__restrict__ parent) {
-  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (v >= n) return;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t v = t / kCcLanes;
+  const int lane = (int)(t % kCcLanes);
+  if (v >= n) return;  // a node's lanes leave together
   int32_t m = (int32_t)v;
-  for (int64_t b = indptr[v]; b < indptr[v + 1]; ++b) m = indices[b] < m ? indices[b] : m;
-  parent[v] = m;
+  for (int64_t b = indptr[v] + lane; b < indptr[v + 1]; b += kCcLanes) m = indices[b] < m ? indices[b] : m;
+#pragma unroll
+  for (int off = kCcLanes / 2; off > 0; off >>= 1) {
+    const int32_t o = __shfl_xor(m, off, kCcLanes);
+    m = o < m ? o : m;
+  }
+  if (lane == 0) parent[v] = m;
 }
 
+// Every edge once, from its larger end; a node's edges are split over its kCcLanes lanes,
+// each with its own (possibly stale) root of v, which a failed CAS refreshes.
 __global__ void k_cc_hook(int64_t n, const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
                           int32_t *parent) {
-  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t v = t / kCcLanes;
+  const int lane = (int)(t % kCcLanes);
   if (v >= n) return;
   int32_t vr = cc_find(parent, (int32_t)v);
-  for (int64_t b = indptr[v]; b < indptr[v + 1]; ++b) {
+  for (int64_t b = indptr[v] + lane; b < indptr[v + 1]; b += kCcLanes) {
     const int32_t u = indices[b];
     if (u >= v) continue;  // every edge once, from its larger end
     int32_t ur = cc_find(parent, u);
@@ -426,8 +441,8 @@ inch_status inch_rcm_device(int64_t n, const int64_t *indptr, const int32_t *ind
   INCH_REQUIRE(!bad_h, "inch_rcm_device: seeds is not a permutation");
   k_rcm_init<<<blocks_for(n, bs), bs, 0, st>>>(n, pos, claim);
   // components, their seeds (first node in the seed order) and sizes, in seed order
-  k_cc_init<<<blocks_for(n, bs), bs, 0, st>>>(n, indptr, indices, label);
-  k_cc_hook<<<blocks_for(n, bs), bs, 0, st>>>(n, indptr, indices, label);
+  k_cc_init<<<blocks_for(n * kCcLanes, bs), bs, 0, st>>>(n, indptr, indices, label);
+  k_cc_hook<<<blocks_for(n * kCcLanes, bs), bs, 0, st>>>(n, indptr, indices, label);
   k_cc_flatten<<<blocks_for(n, bs), bs, 0, st>>>(n, label);
   k_cc_rank<<<blocks_for(n, bs), bs, 0, st>>>(n, seeds, rank, minr);
   k_cc_minrank<<<blocks_for(n, bs), bs, 0, st>>>(n, label, rank, minr);
