@@ -27,6 +27,7 @@ from . import ops
 from ._lib import call
 from .hessian import build_hessian
 from .operator import Operator
+from .trlm import EIGH_MIN_ORDER, _projected_ritz_device  # noqa: F401  (re-exported)
 
 
 class Segments:
@@ -281,53 +282,6 @@ def _projected_ritz(a, b, bk, k, filtered):
     else:
         Sel, W = w[:, :k], Z[:, :, :k]
     return Sel, W, b[-1][:, None] * W[:, -1, :]
-
-
-EIGH_MIN_ORDER = 72   # the device's batched eigh is about 20x faster from order 65 on (measured:
-                      # 512 matrices of order 64 take 209 ms, of order 65 11 ms, of 72 4.6 ms)
-
-
-def _projected_ritz_device(alpha, beta, m, Sel_prev, bk, k, filtered):
-    """_projected_ritz on the device, for the restart at the end of a cycle (the stream is idle
-    there; a batched eigh of 60 projected matrices of order 84 takes 2 ms on the GPU against
-    23 ms on the host). Matrices of order below EIGH_MIN_ORDER are padded to it with a
-    diagonal block of entries beyond the Gershgorin bound on the unwanted side (below the
-    spectrum when the largest pairs are wanted, above it otherwise), whose eigenpairs are
-    never selected and whose rows are dropped. alpha / beta: device [>= m, S]; Sel_prev
-    [S, k] (host) replaces the first k diagonal entries and zeroes the first k betas after a
-    thick restart, bk [S, k] (host) the arrowhead row. Returns (Sel [S, k] host, W [S, m, k]
-    device contiguous, estimates [S, k] host)."""
-    a = alpha[:m].T.contiguous()                   # [S, m]
-    b = beta[:m].T.contiguous()
-    dev = a.device
-    if Sel_prev is not None:
-        a[:, :k] = torch.as_tensor(np.ascontiguousarray(Sel_prev), device=dev)
-        b[:, :k] = 0.0
-    mp = max(m, EIGH_MIN_ORDER)
-    S = a.shape[0]
-    T = torch.zeros((S, mp, mp), dtype=a.dtype, device=dev)
-    idx = torch.arange(m, device=dev)
-    T[:, idx, idx] = a
-    T[:, idx[:-1], idx[1:]] = b[:, :-1]
-    T[:, idx[1:], idx[:-1]] = b[:, :-1]
-    if bk is not None:
-        bkd = torch.as_tensor(np.ascontiguousarray(bk), device=dev)
-        T[:, k, :k] = bkd
-        T[:, :k, k] = bkd
-    if mp > m:
-        g = T[:, :m, :m].abs().sum(-1).amax(-1)      # Gershgorin: every eigenvalue lies in [-g, g]
-        pad = torch.arange(1, mp - m + 1, dtype=a.dtype, device=dev)
-        fill = (g[:, None] + pad[None, :]) * (-1.0 if filtered else 1.0)
-        ip = torch.arange(m, mp, device=dev)
-        T[:, ip, ip] = fill
-    w, Z = torch.linalg.eigh(T)                    # ascending per structure
-    if filtered:
-        Sel, W = w.flip(-1)[:, :k], Z.flip(-1)[:, :m, :k]
-    else:
-        Sel, W = w[:, :k], Z[:, :m, :k]
-    W = W.contiguous()
-    est = b[:, -1:] * W[:, -1, :]
-    return Sel.cpu().numpy(), W, est.cpu().numpy()
 
 
 def solve_segments(A, seg_atoms, k, seg_struct=None, precision="mf64", tol_rel=1e-8, shift=10.0, maxiter=500,
