@@ -210,7 +210,7 @@ inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *st
 /* Cluster layout of the matrix-free operator: another layout of the neighbour CSR that
  * inch_anm_matfree_spmv reads (same reference interfaces). The atoms are grouped into
  * ncl = ceil(n / 4) clusters that are compact in space: cells of cell_xy x cell_xy x cell_z
- * (z cells coarsened until there are at most max(2n, 65536) cells), numbered by x, y column
+ * (z cells coarsened until there are at most max(16n, 65536) cells), numbered by x, y column
  * then z, atoms in a cell by index, cut into groups of 4; the groups are then numbered by
  * their smallest neighbour index (ties by spatial order), so consecutive clusters gather
  * from one band of the vectors; cluster_atoms int32[ncl * 4] (-1 pads the last group). Each cluster lists the union of its
