@@ -46,6 +46,7 @@ SIGNATURES = {
     "inch_bsr_col16": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "inch_stream_copy": (_STATUS, [_i64, _vp, _vp, _vp]),
     "inch_anm_clusters": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "inch_anm_clusters_one_pass": (_STATUS, [_i64, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "inch_anm_matfree_cl_apply": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _dbl, _vp, _vp, _i32, _vp]),
     "inch_anm_records": (_STATUS, [_i64, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "inch_anm_matfree_rec": (_STATUS, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _vp, _dbl, _i32,

@@ -460,7 +460,9 @@ __device__ __forceinline__ double cl_pair_r2(const double *__restrict__ xyz, int
 // ascending order in LDS, and each run of equal j becomes one entry with the OR of its atom
 // bits.
 // Count pass (FILL = false): counts[c] = entries, base[c] = smallest j; overflow bit 1 =
-// more than kMaxUnion row entries, bit 2 = the union spans 2^24 columns or more.
+// more than kMaxUnion row entries, bit 2 = the union spans 2^24 columns or more. With
+// prov_ptr given it also writes the entries, at prov_ptr[c] (room for all of the cluster's row
+// entries), for a later copy into the final layout.
 // Fill pass: entries[cl_ptr[c] + r] = ((j - base[c]) << 8) | mask, ascending j.
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_cl_union(int64_t n, int64_t ncl, const double *__restrict__ xyz,
@@ -469,7 +471,8 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t n, int64_t ncl, const 
                                                    const int32_t *__restrict__ cluster_atoms,
                                                    const int64_t *__restrict__ cl_ptr, int32_t *__restrict__ base,
                                                    int64_t *__restrict__ counts, uint32_t *__restrict__ entries,
-                                                   int32_t *__restrict__ overflow) {
+                                                   int32_t *__restrict__ overflow,
+                                                   const int64_t *__restrict__ prov_ptr = nullptr) {
   __shared__ uint64_t key[kMaxUnion];
   __shared__ int32_t val[kMaxUnion];
   __shared__ int32_t part[256];
@@ -592,6 +595,13 @@ __global__ void __launch_bounds__(256) k_cl_union(int64_t n, int64_t ncl, const 
       base[c] = s_min;
       if (emitted > 0 && (int64_t)s_max - (int64_t)s_min >= ((int64_t)1 << 24)) atomicOr(overflow, 2);
     }
+    if (prov_ptr) {
+      const int64_t out = prov_ptr[c] + before;
+      const uint32_t b0 = (uint32_t)s_min;
+      int r = 0;
+      for (int e = e0; e < e1; ++e)
+        if (val[e]) entries[out + r++] = (((uint32_t)(key[e] >> 3) - b0) << 8) | (uint32_t)val[e];
+    }
     return;
   }
   const int64_t out = cl_ptr[c] + before;
@@ -619,6 +629,31 @@ __global__ void k_cl_permute(int64_t ncl, const uint64_t *__restrict__ perm, con
   for (int t = 0; t < kClusterAtoms; ++t) atoms_out[kClusterAtoms * c + t] = atoms_in[kClusterAtoms * src + t];
   counts_out[c] = counts_in[src];
   base_out[c] = base_in[src];
+}
+
+// tot[c] = entries of the rows of cluster c's atoms (room for its list before the union)
+__global__ void k_cl_totals(int64_t ncl, const int32_t *__restrict__ cluster_atoms,
+                            const int64_t *__restrict__ rowptr, int64_t *__restrict__ tot) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= ncl) return;
+  int64_t s = 0;
+  for (int t = 0; t < kClusterAtoms; ++t) {
+    const int32_t a = cluster_atoms[kClusterAtoms * c + t];
+    if (a >= 0) s += rowptr[a + 1] - rowptr[a];
+  }
+  tot[c] = s;
+}
+
+// Final layout from the count pass's provisional lists: cluster c' (new numbering) takes the
+// list of cluster perm[c'] (old numbering), written at prov_ptr of the old cluster.
+__global__ void __launch_bounds__(256) k_cl_copy(int64_t ncl, const uint64_t *__restrict__ perm,
+                                                 const int64_t *__restrict__ prov_ptr,
+                                                 const uint32_t *__restrict__ prov,
+                                                 const int64_t *__restrict__ cl_ptr, uint32_t *__restrict__ entries) {
+  const int64_t c = blockIdx.x;
+  const int64_t src = prov_ptr[(int64_t)(perm[c] & 0xffffffffu)];
+  const int64_t dst = cl_ptr[c], len = cl_ptr[c + 1] - dst;
+  for (int64_t e = threadIdx.x; e < len; e += blockDim.x) entries[dst + e] = prov[src + e];
 }
 
 // ---- record form of the cluster layout (the operator's Chebyshev recurrence) ----
@@ -938,10 +973,14 @@ extern "C" inch_status inch_hessian_assemble_pbc(int64_t n, const double *xyz, d
   return INCH_OK;
 }
 
-extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
-                                         double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
-                                         int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out,
-                                         void *stream) {
+// The cluster layout, in two calls (one_pass = false: the first, without entries, builds the
+// clusters and their counts; the second, with entries, merges the rows again in the final
+// numbering) or in one (the count pass also writes each list into a provisional buffer with
+// room for the cluster's row entries, and the lists are copied into the final numbering).
+static inch_status anm_clusters_impl(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                     double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
+                                     int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out,
+                                     void *stream, bool one_pass) {
   INCH_REQUIRE(n > 0 && xyz && rowptr && col && cluster_atoms && cl_ptr && base && cell_xy > 0 && cell_z > 0,
                "inch_anm_clusters: bad args");
   INCH_REQUIRE(n < (int64_t)1 << 31, "inch_anm_clusters: n must fit int32");
@@ -961,7 +1000,7 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
     if (over & 4) INCH_FAIL(INCH_ERR_INVALID, "inch_anm_clusters (%s): a row's columns are not in ascending order or outside [0, n)", pass);
     return INCH_OK;
   };
-  if (entries) {  // fill pass on the layout of the count pass
+  if (entries && !one_pass) {  // fill pass on the layout of the count pass
     int64_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1036,8 +1075,21 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   HIP_TRY(hipMalloc(&base0.p, ncl * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&perm.p, ncl * sizeof(uint64_t)));
   HIP_TRY(hipMemcpyAsync(atoms0.p, cluster_atoms, slots * sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+  DevBuf prov_ptr, prov;
+  if (one_pass) {   // room for every cluster's row entries, in the count pass's numbering
+    HIP_TRY(hipMalloc(&prov_ptr.p, (ncl + 1) * sizeof(int64_t)));
+    k_cl_totals<<<(unsigned)((ncl + bs - 1) / bs), bs, 0, st>>>(ncl, (int32_t *)atoms0.p, rowptr,
+                                                               (int64_t *)counts0.p);
+    HIP_TRY(hipGetLastError());
+    SCAN_TRY(ncl, (const int64_t *)counts0.p, (int64_t *)prov_ptr.p);
+    int64_t room = 0;
+    HIP_TRY(hipMemcpyAsync(&room, (int64_t *)prov_ptr.p + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMalloc(&prov.p, std::max<int64_t>(1, room) * sizeof(uint32_t)));
+  }
   k_cl_union<false><<<(unsigned)ncl, 256, 0, st>>>(n, ncl, xyz, rowptr, col, (int32_t *)atoms0.p, nullptr, (int32_t *)base0.p,
-                                                   (int64_t *)counts0.p, nullptr, (int32_t *)ovf.p);
+                                                   (int64_t *)counts0.p, (uint32_t *)prov.p, (int32_t *)ovf.p,
+                                                   (const int64_t *)prov_ptr.p);
   HIP_TRY(hipGetLastError());
   {
     inch_status s0 = check_overflow("count");
@@ -1067,7 +1119,32 @@ extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int
   HIP_TRY(hipMemcpyAsync(&total, cl_ptr + ncl, sizeof(int64_t), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   if (nnz_out) *nnz_out = total;
+  if (one_pass) {
+    if (total > capacity) INCH_FAIL(INCH_ERR_CAPACITY, "inch_anm_clusters_one_pass: need %lld > capacity %lld", (long long)total, (long long)capacity);
+    k_cl_copy<<<(unsigned)ncl, 256, 0, st>>>(ncl, (const uint64_t *)perm.p, (const int64_t *)prov_ptr.p,
+                                             (const uint32_t *)prov.p, cl_ptr, entries);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));   // prov is freed on return
+  }
   return INCH_OK;
+}
+
+extern "C" inch_status inch_anm_clusters(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                         double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
+                                         int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out,
+                                         void *stream) {
+  return anm_clusters_impl(n, xyz, rowptr, col, cell_xy, cell_z, cluster_atoms, cl_ptr, base, entries, capacity,
+                           nnz_out, stream, false);
+}
+
+extern "C" inch_status inch_anm_clusters_one_pass(int64_t n, const double *xyz, const int64_t *rowptr,
+                                                  const int32_t *col, double cell_xy, double cell_z,
+                                                  int32_t *cluster_atoms, int64_t *cl_ptr, int32_t *base,
+                                                  uint32_t *entries, int64_t capacity, int64_t *nnz_out,
+                                                  void *stream) {
+  INCH_REQUIRE(entries && capacity >= 0, "inch_anm_clusters_one_pass: entries needed");
+  return anm_clusters_impl(n, xyz, rowptr, col, cell_xy, cell_z, cluster_atoms, cl_ptr, base, entries, capacity,
+                           nnz_out, stream, true);
 }
 
 extern "C" inch_status inch_anm_records(int64_t n, const double *xyz, int64_t ncl, const int32_t *cluster_atoms,

@@ -232,11 +232,12 @@ def bsr_col16(n_atoms, rowptr, col):
 CLUSTER_ATOMS = 4   # atoms per cluster of the cluster layout (inch_anm_clusters)
 
 
-def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None):
-    """Cluster layout of the matrix-free operator (inch_anm_clusters):
-    (cluster_atoms int32 [ceil(n/4)*4], cl_ptr int64 [ncl+1], base int32 [ncl],
-    entries int32 [nnz] holding the uint32 words ((j - base) << 8) | mask).
-    Raises InchingError (INCH_ERR_CAPACITY) when the layout does not fit the matrix."""
+def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None, one_pass=True):
+    """Cluster layout of the matrix-free operator (inch_anm_clusters_one_pass, or the two calls
+    of inch_anm_clusters with ``one_pass=False``): (cluster_atoms int32 [ceil(n/4)*4],
+    cl_ptr int64 [ncl+1], base int32 [ncl], entries int32 [nnz] holding the uint32 words
+    ((j - base) << 8) | mask). Raises InchingError (INCH_ERR_CAPACITY) when the layout does
+    not fit the matrix."""
     require_gpu()
     _check_dev(xyz)
     n = xyz.shape[0]
@@ -248,6 +249,12 @@ def anm_clusters(xyz, rowptr, col, cell_xy, cell_z, stream=None):
     ptr = torch.empty(ncl + 1, dtype=torch.int64, device=dev)
     base = torch.empty(ncl, dtype=torch.int32, device=dev)
     nnz = C.c_int64(0)
+    if one_pass:
+        room = max(1, int(rowptr[-1].item()))          # the CSR's entries bound the layout's
+        entries = torch.empty(room, dtype=torch.int32, device=dev)
+        call("inch_anm_clusters_one_pass", n, P(xyz), P(rowptr), P(col), float(cell_xy), float(cell_z), P(atoms),
+             P(ptr), P(base), P(entries), room, C.byref(nnz), stream_ptr(stream))
+        return atoms, ptr, base, entries[:nnz.value].clone()
     call("inch_anm_clusters", n, P(xyz), P(rowptr), P(col), float(cell_xy), float(cell_z), P(atoms), P(ptr), P(base),
          None, 0, C.byref(nnz), stream_ptr(stream))
     entries = torch.empty(max(1, nnz.value), dtype=torch.int32, device=dev)

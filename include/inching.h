@@ -224,6 +224,15 @@ inch_status inch_stream_copy(int64_t bytes, const void *src, void *dst, void *st
 inch_status inch_anm_clusters(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
                               double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
                               int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out, void *stream);
+/* The same layout in one call: entries must have room for the layout (capacity >= its nnz;
+ * the neighbour CSR's entry count rowptr[n] always suffices). The count pass writes every
+ * cluster's list into a provisional buffer (rowptr[n] words, allocated and freed here) and
+ * the lists are copied into the final numbering, instead of a second merge of the rows.
+ * Same outputs, bit for bit, as the two calls; the same errors. Syncs. */
+inch_status inch_anm_clusters_one_pass(int64_t n, const double *xyz, const int64_t *rowptr, const int32_t *col,
+                                       double cell_xy, double cell_z, int32_t *cluster_atoms, int64_t *cl_ptr,
+                                       int32_t *base, uint32_t *entries, int64_t capacity, int64_t *nnz_out,
+                                       void *stream);
 /* inch_anm_matfree_spmv on the cluster layout: the same product and fused epilogue, the
  * neighbour's coordinates and vector entries loaded once per cluster (one wave per cluster,
  * lanes over its list). config = list entries per lane iteration (1, 2 or 4; 0 = 2002), or

@@ -531,6 +531,9 @@ def test_cluster_layout_reproduces_csr(mat):
     ref = np.sort(rows[off] * np.int64(n) + col[off])
     assert np.array_equal(got, ref)
     again = ops.anm_clusters(A.xyz, A.rowptr, A.col, 0.5 * A.rc, 0.25 * A.rc)
+    two_calls = ops.anm_clusters(A.xyz, A.rowptr, A.col, 0.5 * A.rc, 0.25 * A.rc, one_pass=False)
+    for a, b in zip(again, two_calls):          # one pass (provisional lists copied) = two merges
+        assert torch.equal(a, b)
     for a, b in zip(cl, again):
         assert torch.equal(a, b)
 
