@@ -348,16 +348,69 @@ class KPMCounter:
     lower end of the spectrum is about dd (pi / degree)^2 / 2 in eigenvalue. The moments
     come from degree / 2 operator launches (fused Chebyshev steps) by the doubling relations
     mu_2k = 2 <T_k v, T_k v> - mu_0 and mu_2k-1 = 2 <T_k v, T_k-1 v> - mu_1 (two dots per
-    step; an odd degree takes one more launch)."""
+    step; an odd degree takes one more launch). With the matrix-free operator the recurrence
+    runs on the cluster-order records (the probe is drawn in the matrix's order, as otherwise,
+    and packed) and the dots on their slot-order outputs, whose padding slots are zero."""
 
     def __init__(self, A, bound, degree=64, probes=1, seed=0, precision="mf64"):
-        from .autotune import best_config, bsr_launch
         if precision != "mf64":
             A.ensure(precision)
         lo, hi = bound
         self.cc, self.dd = 0.5 * (hi + lo), 0.5 * (hi - lo)
         self.n = A.shape[0]
         self.degree = int(degree)
+        dev = A.device
+        half = (self.degree + 1) // 2
+        dots = torch.zeros((probes, 2 * half + 1), dtype=torch.float64, device=dev)
+        g = torch.Generator(device=dev)
+        g.manual_seed(int(seed))
+        ws = ops.Workspace(1, dev)
+        rec = A.records() if precision == "mf64" and getattr(A, "images", None) is None else None
+        if rec is not None:
+            self._dots_on_records(A, rec, g, probes, half, dots, ws)
+        else:
+            self._dots(A, precision, g, probes, half, dots, ws)
+        dh = dots.cpu().numpy()
+        mom = np.empty((probes, self.degree + 1))
+        mom[:, 0] = 1.0
+        mom[:, 1] = dh[:, 1]
+        for k in range(1, half + 1):
+            if 2 * k <= self.degree:
+                mom[:, 2 * k] = 2.0 * dh[:, 2 * k] - 1.0
+            if 2 <= 2 * k - 1 <= self.degree:
+                mom[:, 2 * k - 1] = 2.0 * dh[:, 2 * k - 1] - mom[:, 1]
+        self.moments = mom.mean(0)
+        self.jackson = jackson_kernel(self.degree)
+        self.n_spmv = probes * half
+
+    def _dots_on_records(self, A, rec, g, probes, half, dots, ws):
+        """dots[p, 2k] = <T_k v, T_k v>, dots[p, 2k - 1] = <T_k v, T_k-1 v> for probe p, the
+        recurrence on three record buffers in turn, T_k v in slot order from each step."""
+        recs, rent = rec
+        atoms, ptr = A.clusters()[:2]
+        ncl = ptr.numel() - 1
+        ys = [torch.empty(12 * ncl, dtype=torch.float64, device=A.device) for _ in range(2)]
+        for p in range(probes):
+            v = torch.randn(self.n, generator=g, dtype=torch.float64, device=A.device)
+            v /= torch.linalg.norm(v)
+            ops.rec_pack(atoms, v, recs[0], acc=ys[0], mu0=1.0)          # T_0 v: records and slot order
+            prev, cur, free = None, recs[0], [recs[1], recs[2]]
+            for k in range(1, half + 1):
+                nxt = free.pop(0)
+                y, y_prev = ys[k % 2], ys[(k - 1) % 2]
+                if k == 1:
+                    ops.matfree_rec(cur, None, nxt, ptr, rent, A.plus_i, s1=1.0 / self.dd, c0=self.cc, y=y)
+                else:
+                    ops.matfree_rec(cur, prev, nxt, ptr, rent, A.plus_i, s1=2.0 / self.dd, c0=self.cc, c2=1.0, y=y)
+                ops.multidot(y.view(1, -1), 1, y, dots[p, 2 * k:2 * k + 1], ws)
+                ops.multidot(y_prev.view(1, -1), 1, y, dots[p, 2 * k - 1:2 * k], ws)
+                if prev is not None:
+                    free.append(prev)
+                prev, cur = cur, nxt
+
+    def _dots(self, A, precision, g, probes, half, dots, ws):
+        """The same dots with the recurrence in the matrix's order (any operator precision)."""
+        from .autotune import best_config, bsr_launch
         dev = A.device
         vdt = torch.float32 if precision == "f32" else torch.float64
         cfg = best_config(A, precision)
@@ -370,14 +423,9 @@ class KPMCounter:
             else:
                 ops.bsr_spmv(A.n_atoms, A.rowptr, A.col, vals, x, y, lanes=blanes, col16=c16, **epi)
 
-        g = torch.Generator(device=dev)
-        g.manual_seed(int(seed))
-        ws = ops.Workspace(1, dev)
         t = [torch.empty(self.n, dtype=vdt, device=dev) for _ in range(3)]
         d64 = torch.empty(self.n, dtype=torch.float64, device=dev) if vdt != torch.float64 else None
-        half = (self.degree + 1) // 2
         d64p = torch.empty(self.n, dtype=torch.float64, device=dev) if vdt != torch.float64 else None
-        dots = torch.zeros((probes, 2 * half + 1), dtype=torch.float64, device=dev)
         for p in range(probes):
             v = torch.randn(self.n, generator=g, dtype=torch.float64, device=dev)
             v /= torch.linalg.norm(v)
@@ -403,18 +451,6 @@ class KPMCounter:
                     prev64 = src
                 free = prev if prev is not None else t[2]
                 prev, cur, nxt = cur, nxt, free
-        dh = dots.cpu().numpy()
-        mom = np.empty((probes, self.degree + 1))
-        mom[:, 0] = 1.0
-        mom[:, 1] = dh[:, 1]
-        for k in range(1, half + 1):
-            if 2 * k <= self.degree:
-                mom[:, 2 * k] = 2.0 * dh[:, 2 * k] - 1.0
-            if 2 <= 2 * k - 1 <= self.degree:
-                mom[:, 2 * k - 1] = 2.0 * dh[:, 2 * k - 1] - mom[:, 1]
-        self.moments = mom.mean(0)
-        self.jackson = jackson_kernel(self.degree)
-        self.n_spmv = probes * half
 
     def count(self, r):
         x = min(1.0, max(-1.0, (float(r) - self.cc) / self.dd))
