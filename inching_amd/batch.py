@@ -205,8 +205,7 @@ class RigidBody:
         for l, field in enumerate(((zero, -r[:, 2], r[:, 1]), (r[:, 2], zero, -r[:, 0]), (-r[:, 1], r[:, 0], zero))):
             so.rigid(self, torch.stack(field, 1).reshape(-1), None, 0.0)
             K[:, :, l] = self.g.view(S, 6)[:, 3:]
-        # the S 3 x 3 inverses on the host: the device's batched small-matrix routines run
-        # one matrix at a time
+        # the S 3 x 3 determinants and inverses on the host (numpy; S x 9 numbers)
         Kh = K.cpu().numpy()
         Kh = 0.5 * (Kh + Kh.transpose(0, 2, 1))
         if np.any(np.abs(np.linalg.det(Kh)) <= 1e-12 * np.linalg.norm(Kh, axis=(1, 2)) ** 3):
